@@ -1,0 +1,199 @@
+"""Benchmark: volumes/s of the RF -> volume path on one MI355X.
+
+Workload (default): 128-element row-column array, 128 Hadamard-encoded
+transmits x 128 channels x 4096 int16 samples (128 MiB raw frame, already in
+HBM), channel remap + int8-MFMA decode, 64-tap match filter + Hilbert, HERCULES
+3-D DAS to 128^3 complex64 with linear interpolation, F# 1.5 Hann.  One step
+is one full pass raw -> volume.  Prints one JSON line.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+FLOP_PER_TAP = {"linear": 34, "cubic": 65}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(acq, interp):
+    """FP64 oracle on a bounded sample of the same workload, scaled to volumes/s."""
+    import numpy as np
+
+    import oracle
+    from cuda_ultrasound_toolkit_amd import _lib as L
+    oracle.load()
+    bp = acq.bp
+    S, Cc, T = bp.dec_data_dim[0], bp.dec_data_dim[1], bp.dec_data_dim[2]
+    csub = 8
+    t0 = time.perf_counter()
+    conv = oracle.convert(acq.raw, L.TYPE_I16, np.array(bp.channel_mapping[:csub], np.int16), bp.rf_raw_dim[0], T, csub, S)
+    dec = oracle.decode(conv, oracle.hadamard(T, bp.decode))
+    t_dec = (time.perf_counter() - t0) * (Cc / csub)
+    rows = dec.reshape(T * csub, S)[:1024]
+    h = np.array(bp.rf_filter[: bp.filter_length], np.float64)
+    t0 = time.perf_counter()
+    oracle.hilbert(rows, h)
+    t_hil = (time.perf_counter() - t0) * (T * Cc / 1024)
+    # DAS on 4 of the 128 depth planes (iz = 16, 48, 80, 112), full IQ
+    sub = L.CudaBeamformerParameters.from_buffer_copy(bp)
+    nz = bp.output_points[2]
+    zmin, zmax = bp.output_min_coordinate[2], bp.output_max_coordinate[2]
+    res = (zmax - zmin) / (nz - 1)
+    sub.output_points[2] = 4
+    sub.output_min_coordinate[2] = zmin + 16 * res
+    sub.output_max_coordinate[2] = zmin + 112 * res
+    rng = np.random.default_rng(0)
+    iq = (rng.standard_normal((T, Cc, S)) + 1j * rng.standard_normal((T, Cc, S)))
+    t0 = time.perf_counter()
+    oracle.das(iq, sub, cubic=(interp == "cubic"))
+    t_das = (time.perf_counter() - t0) * (nz / 4)
+    total = t_dec + t_hil + t_das
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": 1.0 / total, "unit": "volumes/s", "cores": cores, "kind": "port",
+            "sample": (f"FP64 oracle: decode of {csub}/{Cc} channels, Hilbert of 1024/{T * Cc} rows, "
+                       f"DAS of 4/{nz} depth planes, each scaled to the full frame; "
+                       f"stage s/volume decode {t_dec:.2f}, hilbert {t_hil:.2f}, das {t_das:.2f}")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--interp", choices=["linear", "cubic"], default="linear")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--stage-iters", type=int, default=10)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); frames shard with no exchange, "
+              "see the sharding notes, but no multi-GPU figure is produced.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from cuda_ultrasound_toolkit_amd import _lib as L, api, configs
+
+    lib = L.load()
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    log("bench: synthesizing 128 tx x 128 ch x 4096 frame")
+    acq = configs.rca_3d(seed=2)
+    bp = acq.bp
+    S, Cc, T = bp.dec_data_dim[0], bp.dec_data_dim[1], bp.dec_data_dim[2]
+    nx, ny, nz = bp.output_points[0], bp.output_points[1], bp.output_points[2]
+    d_raw = torch.from_numpy(acq.raw).to(dev)
+    d_vol = torch.zeros(nx * ny * nz * 2, dtype=torch.float32, device=dev)
+    stream = torch.cuda.current_stream()
+    api.set_stream(stream.cuda_stream)
+    assert api.set_das_options(cubic=(args.interp == "cubic"))
+
+    log("bench: warmup")
+    for _ in range(args.warmup):
+        assert api.beamform_device(d_raw, d_vol, bp)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t_wall = time.perf_counter()
+    e0.record(stream)
+    for _ in range(args.steps):
+        assert api.beamform_device(d_raw, d_vol, bp)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_wall
+    ms = e0.elapsed_time(e1) / args.steps
+    log(f"bench: {ms:.3f} ms/volume (wall {1000 * wall / args.steps:.3f})")
+
+    # ---- per-stage timing on the same data (stage entry points, same stream)
+    def timed(fn, iters):
+        fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(iters):
+            fn()
+        b.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / iters
+
+    assert api.rf_init((bp.rf_raw_dim[0], bp.rf_raw_dim[1]), (S, Cc, T), bp.decode)
+    assert api.rf_set_channel_mapping(np.array(bp.channel_mapping[:Cc], np.int16))
+    assert api.rf_set_match_filter(np.array(bp.rf_filter[: bp.filter_length], np.float32))
+    d_dec = torch.zeros(T * Cc * S, dtype=torch.float32, device=dev)
+    d_iq = torch.zeros(T * Cc * S * 2, dtype=torch.float32, device=dev)
+    it = args.stage_iters
+    ms_dec = timed(lambda: api.rf_convert_decode(d_raw, L.TYPE_I16, d_dec, 1, True), it)
+    ms_hil = timed(lambda: api.rf_hilbert(d_dec, 1, d_iq), it)
+    ms_das = timed(lambda: api.das_device(d_iq, d_vol, bp), max(3, it // 2))
+    api.set_decode_mode(1)
+    ms_dec_f32 = timed(lambda: api.rf_convert_decode(d_raw, L.TYPE_I16, d_dec, 1, True), it)
+    api.set_decode_mode(0)
+    other_interp = "cubic" if args.interp == "linear" else "linear"
+    api.set_das_options(cubic=(other_interp == "cubic"))
+    ms_das_other = timed(lambda: api.das_device(d_iq, d_vol, bp), 3)
+    api.set_das_options(cubic=(args.interp == "cubic"))
+    api.set_stream(None)
+
+    taps = C.c_ulonglong()
+    assert lib.cut_das_active_taps(C.byref(bp), C.byref(taps))
+    tcs = T * Cc * S
+    das_tflops = taps.value * FLOP_PER_TAP[args.interp] / (ms_das * 1e-3) / 1e12
+    das_tflops_other = taps.value * FLOP_PER_TAP[other_interp] / (ms_das_other * 1e-3) / 1e12
+    dec_gbs = (2 + 4) * tcs / (ms_dec * 1e-3) / 1e9
+    hil_gbs = (4 + 8) * tcs / (ms_hil * 1e-3) / 1e9
+
+    traffic = None
+    tfile = ROOT / "profiles" / "traffic.json"
+    if tfile.exists():
+        traffic = json.loads(tfile.read_text()).get(f"das_{args.interp}")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("bench: cpu baseline (FP64 oracle sample)")
+        cpu = cpu_baseline(acq, args.interp)
+
+    value = 1000.0 / ms
+    out = {
+        "metric": "volumes/s at 128ch x 128tx -> 128^3 (raw int16 in HBM -> complex64 volume)",
+        "value": round(value, 3),
+        "unit": "volumes/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded point-scatterer RCA frame, Hadamard-128 encoded)",
+        "config": {"workload": "rca_3d: 128 tx x 128 ch x 4096 int16 -> decode -> 64-tap filter + Hilbert "
+                               f"-> HERCULES DAS 128^3, {args.interp} interpolation, F# 1.5 Hann",
+                   "interp": args.interp, "decode": "int8 MFMA hi/lo (bit-exact)"},
+        "roofline": {"bound": "valu", "kernel": "hercules_kernel (DAS)",
+                     "achieved": round(das_tflops, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(das_tflops / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
+                     "work": f"{taps.value} active taps x {FLOP_PER_TAP[args.interp]} FLOP",
+                     "kernel_ms": round(ms_das, 4)},
+        "stages_ms": {"decode_i8": round(ms_dec, 4), "decode_f32": round(ms_dec_f32, 4),
+                      "hilbert": round(ms_hil, 4), f"das_{args.interp}": round(ms_das, 4),
+                      f"das_{other_interp}": round(ms_das_other, 4)},
+        "stages_roofline": {"decode_i8_GBs": round(dec_gbs, 1), "decode_i8_frac_hbm": round(dec_gbs / PEAK_HBM_GBS, 4),
+                            "hilbert_GBs": round(hil_gbs, 1), "hilbert_frac_hbm": round(hil_gbs / PEAK_HBM_GBS, 4),
+                            f"das_{other_interp}_TFLOPs": round(das_tflops_other, 3)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,133 @@
+"""Python mirror of the toolkit's entry points.
+
+Same names, argument meaning and error behaviour as the reference
+(cuda_toolkit/src/public/cuda_toolkit.hpp:17-29, cuda_toolkit_ogl.h:37-75):
+each call returns False (after the library prints the reason on stderr)
+instead of raising.  Host arrays are numpy; device arrays are any object with
+``data_ptr()`` (torch tensors on the GPU), used only as plumbing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+
+
+def _span(a: np.ndarray) -> L.Span:
+    if not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("arrays passed to the library must be C-contiguous")
+    return L.Span(a.ctypes.data, a.nbytes)
+
+
+def beamform(input_data: np.ndarray, output_data: np.ndarray, bp: L.CudaBeamformerParameters) -> bool:
+    """cuda_toolkit::beamform: raw RF bytes in, complex64 volume bytes out (host memory)."""
+    lib = L.load()
+    return bool(getattr(lib, L.SYM_BEAMFORM)(_span(input_data), _span(output_data), C.byref(bp)))
+
+
+def beamform_volume(raw: np.ndarray, bp: L.CudaBeamformerParameters) -> np.ndarray | None:
+    """Convenience wrapper: returns the [Nz][Ny][Nx] complex64 volume or None."""
+    nx, ny, nz = bp.output_points[0], bp.output_points[1], bp.output_points[2]
+    out = np.zeros((nz, ny, nx), dtype=np.complex64)
+    return out if beamform(np.ascontiguousarray(raw), out, bp) else None
+
+
+# ---- C API over GL buffers --------------------------------------------------
+def _uints(v):
+    arr = (C.c_uint * len(v))(*v)
+    return arr
+
+
+def init_cuda_configuration(input_dims, decoded_dims) -> bool:
+    return bool(L.load().init_cuda_configuration(_uints(input_dims), _uints(decoded_dims)))
+
+
+def register_cuda_buffers(rf_ssbos, raw_ssbo: int) -> bool:
+    return bool(L.load().register_cuda_buffers(_uints(rf_ssbos), len(rf_ssbos), raw_ssbo))
+
+
+def cuda_set_channel_mapping(mapping: np.ndarray) -> bool:
+    m = np.zeros(L.MAX_CHANNEL_COUNT, np.int16)
+    m[: len(mapping)] = mapping
+    return bool(L.load().cuda_set_channel_mapping(m.ctypes.data_as(C.POINTER(C.c_int16))))
+
+
+def cuda_set_match_filter(taps: np.ndarray | None) -> bool:
+    if taps is None or len(taps) == 0:
+        return bool(L.load().cuda_set_match_filter(None, 0))
+    h = np.ascontiguousarray(taps, dtype=np.float32)
+    return bool(L.load().cuda_set_match_filter(h.ctypes.data_as(C.POINTER(C.c_float)), len(h)))
+
+
+def cuda_decode(input_byte_offset: int, output_buffer_idx: int) -> bool:
+    return bool(L.load().cuda_decode(input_byte_offset, output_buffer_idx))
+
+
+def cuda_hilbert(input_buffer_idx: int, output_buffer_idx: int) -> bool:
+    return bool(L.load().cuda_hilbert(input_buffer_idx, output_buffer_idx))
+
+
+def deinit_cuda_configuration() -> None:
+    L.load().deinit_cuda_configuration()
+
+
+def register_device_buffers(rf_tensors, raw_tensor) -> bool:
+    """GL seam: register device tensors in place of GL buffers."""
+    n = len(rf_tensors)
+    ptrs = (C.c_void_p * max(n, 1))(*[t.data_ptr() for t in rf_tensors])
+    sizes = (C.c_size_t * max(n, 1))(*[t.numel() * t.element_size() for t in rf_tensors])
+    return bool(L.load().cut_gl_register_device_buffers(ptrs, sizes, n, raw_tensor.data_ptr(),
+                                                         raw_tensor.numel() * raw_tensor.element_size()))
+
+
+# ---- extension entry points on device memory --------------------------------
+def set_stream(stream_handle: int | None) -> None:
+    L.load().cut_set_stream(stream_handle)
+
+
+def synchronize() -> bool:
+    return bool(L.load().cut_synchronize())
+
+
+def set_decode_mode(mode: int) -> bool:
+    return bool(L.load().cut_set_decode_mode(mode))
+
+
+def set_das_options(cubic: bool = True, cf_hercules: bool = False, iq_format: int = 0) -> bool:
+    return bool(L.load().cut_set_das_options(int(cubic), int(cf_hercules), iq_format))
+
+
+def beamform_device(d_raw, d_volume, bp: L.CudaBeamformerParameters) -> bool:
+    return bool(L.load().cut_beamform_device(d_raw.data_ptr(), d_raw.numel() * d_raw.element_size(),
+                                             d_volume.data_ptr(), d_volume.numel() * d_volume.element_size(),
+                                             C.byref(bp)))
+
+
+def rf_init(raw_dims, dec_dims, encoding: int) -> bool:
+    return bool(L.load().cut_rf_init(_uints(raw_dims), _uints(dec_dims), encoding))
+
+
+def rf_set_channel_mapping(mapping: np.ndarray) -> bool:
+    m = np.ascontiguousarray(mapping, dtype=np.int16)
+    return bool(L.load().cut_rf_set_channel_mapping(m.ctypes.data_as(C.POINTER(C.c_int16)), len(m)))
+
+
+def rf_set_match_filter(taps) -> bool:
+    if taps is None or len(taps) == 0:
+        return bool(L.load().cut_rf_set_match_filter(None, 0))
+    h = np.ascontiguousarray(taps, dtype=np.float32)
+    return bool(L.load().cut_rf_set_match_filter(h.ctypes.data_as(C.POINTER(C.c_float)), len(h)))
+
+
+def rf_convert_decode(d_raw, dtype: int, d_out, out_stride: int = 1, decode: bool = True) -> bool:
+    return bool(L.load().cut_rf_convert_decode(d_raw.data_ptr(), dtype, d_out.data_ptr(), out_stride, int(decode)))
+
+
+def rf_hilbert(d_in, in_stride: int, d_out) -> bool:
+    return bool(L.load().cut_rf_hilbert(d_in.data_ptr(), in_stride, d_out.data_ptr()))
+
+
+def das_device(d_iq, d_volume, bp: L.CudaBeamformerParameters) -> bool:
+    return bool(L.load().cut_das_device(d_iq.data_ptr(), d_volume.data_ptr(), C.byref(bp)))

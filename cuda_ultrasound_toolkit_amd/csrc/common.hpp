@@ -1,0 +1,134 @@
+// Shared host-side helpers: error convention, device buffers, kernel-side
+// parameter blocks.  Error convention follows the reference
+// (cuda_toolkit/src/defs.h:117-178): failing calls print one line on std::cerr
+// and the entry point returns false.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <iostream>
+#include <vector>
+
+#include "../../include/cuda_beamformer_parameters.h"
+
+#define CUT_HIP_CHECK(expr)                                                              \
+    do {                                                                                 \
+        hipError_t cut_err_ = (expr);                                                    \
+        if (cut_err_ != hipSuccess) {                                                    \
+            std::cerr << "HIP error " << hipGetErrorName(cut_err_) << " '"              \
+                      << hipGetErrorString(cut_err_) << "' at " << __FILE__ << ':'       \
+                      << __LINE__ << std::endl;                                          \
+            return false;                                                                \
+        }                                                                                \
+    } while (0)
+
+#define CUT_FAIL(msg)                                 \
+    do {                                              \
+        std::cerr << "cuda_toolkit: " << msg << std::endl; \
+        return false;                                 \
+    } while (0)
+
+namespace cut {
+
+// Device allocation reused while the requested size is unchanged (the
+// reference's size-keyed buffer cache, cuda_toolkit/src/api/cuda_toolkit.cpp:43-82).
+struct DeviceBuffer {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    DeviceBuffer() = default;
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    ~DeviceBuffer() { release(); }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+    bool ensure(size_t n) {
+        if (n == bytes && ptr) return true;
+        release();
+        if (n == 0) return true;
+        hipError_t e = hipMalloc(&ptr, n);
+        if (e != hipSuccess) {
+            std::cerr << "cuda_toolkit: hipMalloc(" << n << ") failed: " << hipGetErrorString(e) << std::endl;
+            ptr = nullptr;
+            return false;
+        }
+        bytes = n;
+        return true;
+    }
+    template <typename T> T* as() const { return static_cast<T*>(ptr); }
+};
+
+// Library stream: every entry point enqueues on it.  Callers may substitute
+// their own stream (cut_set_stream) so that their events bracket our kernels.
+hipStream_t stream();
+void set_stream(hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// Kernel launchers (defined in the .hip files).
+// ---------------------------------------------------------------------------
+
+// Convert (cuda_toolkit/src/rf_processing/data_conversion/data_conversion_kernels.cuh:25-41):
+// out[(t*C + c)*S + s] * out_stride = raw[map[c]*R0 + t*S + s].
+bool launch_convert(const void* raw, int type, const int16_t* d_map, size_t R0, int T, int C, int S,
+                    float* out, int out_stride, hipStream_t st);
+
+// Decode (hadamard_decoder.cpp:49-78): out[j][c][s] = scale * sum_k H[j][k] raw[map[c]][k*S+s].
+// mode 0: FP32 MFMA GEMM (any input type); mode 1: int8 MFMA on hi/lo byte
+// splits of int16 input with int32 accumulation.  d_h8: int8 H padded to
+// Kp x Kp (Kp = T rounded up to 32); d_hf: float H, same padding; d_rowsum:
+// int32 row sums of H (length Kp).
+bool launch_decode(const void* raw, int type, const int16_t* d_map, size_t R0, int T, int C, int S,
+                   const int8_t* d_h8, const float* d_hf, const int* d_rowsum, float scale,
+                   float* out, int out_stride, int mode, hipStream_t st);
+
+// Hilbert transform with optional match filter (hilbert_handler.cu:132-155).
+// Input row r of length S is read from a float array at in[(r*S + s)*in_stride]
+// (in_kind 0), or, convert fused, from raw int16/f32 with channel map
+// (in_kind 1 = i16, 2 = f32; row r = t*C + c).
+struct HilbertInput {
+    const void* ptr;
+    int kind;        // 0 float strided, 1 raw i16, 2 raw f32
+    int stride;      // kind 0
+    const int16_t* map;
+    size_t R0;
+    int C;
+};
+bool launch_twiddles(float2* d_tw, int S, hipStream_t st);
+bool launch_filter_spectrum(const float* d_h, int L, int S, float2* d_F, hipStream_t st);
+bool launch_hilbert(const HilbertInput& in, int rows, int S, const float2* d_tw, const float2* d_F,
+                    float2* out, hipStream_t st);
+bool hilbert_supported(int S);
+
+// DAS parameters in kernel form (beamformer.cu:17-98).
+struct DasParams {
+    int S, C, T;
+    int Nx, Ny, Nz;
+    float min_x, min_y, min_z;
+    float res_x, res_y, res_z;
+    float xdc_min_x, xdc_min_y, xdc_max_x, xdc_max_y;
+    float pitch_x, pitch_y;
+    float spm;
+    int delay;
+    float focal_z;
+    int focus_type;  // 0 none, 1 XZ_PLANE, 2 YZ_PLANE, 3 XZ_FOCUS, 4 YZ_FOCUS
+    float fn_tx, fn_rx, cw;
+    int apo_type, to_power;
+    int readi_groups, encoding;
+    unsigned long long readi_bits;
+    int sequence;
+};
+bool das_params_from_bp(const CudaBeamformerParameters& bp, DasParams& p);
+
+// iq_kind: 0 complex f32, 1 half2 (scaled by iq_scale), 2 bf16x2 (scaled).
+struct DasOptions {
+    int cubic = 1;         // 1 = Catmull-Rom (reference), 0 = linear
+    int cf_hercules = 0;   // apply the coherence factor in HERCULES as well
+};
+bool launch_das(const void* iq, int iq_kind, float iq_scale, const DasParams& p, const DasOptions& o,
+                float2* vol, hipStream_t st);
+
+}  // namespace cut

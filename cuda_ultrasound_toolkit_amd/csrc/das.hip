@@ -1,0 +1,384 @@
+// Delay-and-sum beamforming kernels for gfx950.
+//
+// HERCULES (3-D, row-column array): templated_beamformer.cuh:102-216.
+// FORCES   (2-D synthetic aperture): templated_beamformer.cuh:219-324.
+// Helpers: beamformer_utils.cuh:99-213, templated_beamformer.cuh:16-99.
+//
+// HERCULES design.  A workgroup owns a 16 x 16 x 8 voxel tile; each thread owns
+// one (x, y) column and its 8 voxels along z, so per (t, c) the lateral
+// distance rho^2 = dx^2 + dy^2 (and, for Hann apodization, rho itself) is
+// computed once and shared by 8 voxels.  For every transmit the workgroup
+// derives, from the tile's bounding box and the F-number aperture, the channel
+// range that can have apo > 0.1 anywhere in the tile; channels outside are
+// skipped without touching memory.  Channels inside are processed 16 at a
+// time: for each, the span of sample indices any voxel of the tile can read is
+// bounded from the box (min/max receive and transmit path), and that window
+// (<= 256 complex samples) is copied into LDS, so the per-tap interpolation
+// reads LDS instead of gathering from L2.  A window that does not fit falls
+// back to direct global reads for that channel.
+#include "common.hpp"
+
+#include <cmath>
+
+namespace cut {
+
+constexpr int HT_X = 16, HT_Y = 16, HT_Z = 8;  // voxel tile
+constexpr int HCB = 16;                         // channels staged per pass
+constexpr int HW = 256;                         // window length (complex samples)
+// apo > 0.1  <=>  fn*rho/(2z) < acos(sqrt(0.1))/pi for the Hann window.
+constexpr float HANN_A_THR = 0.39758361765043326f;
+
+__device__ __forceinline__ float cos_2pi(float x) { return __builtin_amdgcn_cosf(x); }
+
+// cos^2(pi * min(a, 0.5)) = 0.5 + 0.5 cos(2 pi min(a, 0.5))  (beamformer_utils.cuh:142-154)
+__device__ __forceinline__ float hann_from_a(float a)
+{
+    a = fminf(a, 0.5f);
+    return fmaf(0.5f, cos_2pi(a), 0.5f);
+}
+
+// templated_beamformer.cuh:86-99
+__device__ __forceinline__ float sin_apo_to(float pos, float peak, int power)
+{
+    float a = 3.14159265358979f * pos / peak;
+    a = fminf(fmaxf(a, -3.14159265358979f), 3.14159265358979f);
+    a = sinf(a);
+    float out = 1.0f;
+    for (int i = 0; i < power; ++i) out *= a;
+    return out;
+}
+
+__device__ __forceinline__ float2 lerp2(float2 p0, float2 p1, float t)
+{
+    return make_float2(fmaf(t, p1.x - p0.x, p0.x), fmaf(t, p1.y - p0.y, p0.y));
+}
+
+// Catmull-Rom with tangents (p[i+1]-p[i-1])/2 (beamformer_utils.cuh:99-125).
+__device__ __forceinline__ float2 cubic2(float2 pm1, float2 p0, float2 p1, float2 p2, float t)
+{
+    const float t2 = t * t;
+    const float h00 = fmaf(fmaf(2.0f, t, -3.0f), t2, 1.0f);
+    const float h10 = fmaf(t - 2.0f, t2, t);
+    const float h01 = fmaf(-2.0f, t, 3.0f) * t2;
+    const float h11 = (t - 1.0f) * t2;
+    const float m0x = 0.5f * (p1.x - pm1.x), m0y = 0.5f * (p1.y - pm1.y);
+    const float m1x = 0.5f * (p2.x - p0.x), m1y = 0.5f * (p2.y - p0.y);
+    return make_float2(fmaf(h00, p0.x, fmaf(h01, p1.x, fmaf(h10, m0x, h11 * m1x))),
+                       fmaf(h00, p0.y, fmaf(h01, p1.y, fmaf(h10, m0y, h11 * m1y))));
+}
+
+// Read p[i] of a row, index clamped into [0, S-1]: the reference's cubic form
+// touches p[S] at x = S-2 with weight exactly 0, any finite value works.
+__device__ __forceinline__ float2 row_at(const float2* __restrict__ row, int i, int S)
+{
+    i = min(max(i, 0), S - 1);
+    return row[i];
+}
+
+template <bool CUBIC>
+__device__ __forceinline__ float2 interp_global(const float2* __restrict__ row, float x, int S)
+{
+    const float fi = floorf(x);
+    const float t = x - fi;
+    const int i = (int)fi;
+    if constexpr (CUBIC)
+        return cubic2(row_at(row, i - 1, S), row_at(row, i, S), row_at(row, i + 1, S), row_at(row, i + 2, S), t);
+    else
+        return lerp2(row_at(row, i, S), row_at(row, i + 1, S), t);
+}
+
+struct MinMax {
+    float lo, hi;
+};
+
+__device__ __forceinline__ MinMax dist_range(float e, float a, float b)
+{
+    // |e - v| for v in [a, b]
+    const float dmin = (e < a) ? a - e : (e > b ? e - b : 0.0f);
+    const float dmax = fmaxf(fabsf(e - a), fabsf(e - b));
+    return {dmin, dmax};
+}
+
+template <bool CUBIC, bool FOCUS, bool CF, bool SINAPO>
+__global__ __launch_bounds__(256) void hercules_kernel(const float2* __restrict__ iq, const DasParams p,
+                                                       float2* __restrict__ vol)
+{
+    __shared__ float2 win[HCB * HW];
+    __shared__ int wlo_s[HCB];
+    __shared__ int wmode_s[HCB];
+
+    const int tid = threadIdx.x;
+    const int lx = tid % HT_X, ly = tid / HT_X;
+    const int bx = blockIdx.x * HT_X, by = blockIdx.y * HT_Y, bz = blockIdx.z * HT_Z;
+    const int ix = bx + lx, iy = by + ly;
+    const bool col_valid = ix < p.Nx && iy < p.Ny;
+
+    const float vx = p.min_x + (float)min(ix, p.Nx - 1) * p.res_x;
+    const float vy = p.min_y + (float)min(iy, p.Ny - 1) * p.res_y;
+
+    // tile bounding box
+    const float xa = p.min_x + (float)bx * p.res_x, xb = p.min_x + (float)min(bx + HT_X - 1, p.Nx - 1) * p.res_x;
+    const float ya = p.min_y + (float)by * p.res_y, yb = p.min_y + (float)min(by + HT_Y - 1, p.Ny - 1) * p.res_y;
+    const float za = p.min_z + (float)bz * p.res_z, zb = p.min_z + (float)min(bz + HT_Z - 1, p.Nz - 1) * p.res_z;
+    const float X0 = fminf(xa, xb), X1 = fmaxf(xa, xb);
+    const float Y0 = fminf(ya, yb), Y1 = fmaxf(ya, yb);
+    const float Z0 = fminf(za, zb), Z1 = fmaxf(za, zb);
+    const MinMax zr = dist_range(0.0f, Z0, Z1);  // |z| range
+
+    const float fz = p.focal_z;
+    // per-voxel constants
+    float z2[HT_Z], kz[HT_Z], base[HT_Z];
+    bool zvalid[HT_Z];
+#pragma unroll
+    for (int j = 0; j < HT_Z; ++j) {
+        const int iz = bz + j;
+        zvalid[j] = iz < p.Nz;
+        const float z = p.min_z + (float)min(iz, p.Nz - 1) * p.res_z;
+        z2[j] = z * z;
+        kz[j] = p.fn_rx / z * 0.5f;
+        float txd;
+        if constexpr (FOCUS) {
+            const float ty = vy, tzz = z - fz;
+            txd = copysignf(sqrtf(ty * ty + tzz * tzz), tzz) + fz;
+        } else {
+            txd = z;  // (0, y sin 0, z cos 0): steering angle zero
+        }
+        base[j] = (txd + fz) * p.spm + (float)p.delay;
+    }
+    // transmit path bounds over the tile
+    float txd_lo, txd_hi;
+    if constexpr (FOCUS) {
+        const MinMax yr = dist_range(0.0f, Y0, Y1);
+        float cand[4];
+        int n = 0;
+        for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 2; ++b) {
+                const float yy = a ? yr.hi : yr.lo;
+                const float zz = (b ? Z1 : Z0) - fz;
+                cand[n++] = copysignf(sqrtf(yy * yy + zz * zz), zz) + fz;
+            }
+        txd_lo = fminf(fminf(cand[0], cand[1]), fminf(cand[2], cand[3]));
+        txd_hi = fmaxf(fmaxf(cand[0], cand[1]), fmaxf(cand[2], cand[3]));
+    } else {
+        txd_lo = Z0;
+        txd_hi = Z1;
+    }
+
+    // Aperture radius (Hann): rho < 2*A*z/fn.  Unbounded when fn <= 0 or z <= 0.
+    const bool aperture_bounded = !SINAPO && p.fn_rx > 0.0f && Z0 > 0.0f && p.pitch_x > 0.0f;
+    const float rmax_ap = aperture_bounded ? 2.0f * (HANN_A_THR * 1.001f) * Z1 / p.fn_rx + 1e-6f : 0.0f;
+
+    float2 acc[HT_Z];
+    float inc[HT_Z];
+#pragma unroll
+    for (int j = 0; j < HT_Z; ++j) { acc[j] = make_float2(0.0f, 0.0f); inc[j] = 0.0f; }
+
+    const int G = p.readi_groups > 1 ? p.readi_groups : 1;
+    const float lo_clamp = 1.0f, hi_clamp = (float)p.S - 2.0f;
+    const size_t rowlen = (size_t)p.S;
+
+    for (int g = 0; g < G; ++g) {
+        for (int t = 0; t < p.T; ++t) {
+            float cur = (float)t;
+            bool neg = false;
+            if (G > 1) {
+                cur = p.encoding == WALSH ? (float)t * (float)G + (float)g : (float)g * (float)p.T + (float)t;
+                const int bit = p.encoding == WALSH ? ((t & 1) ? G - 1 - g : g) : g;
+                neg = (p.readi_bits >> bit) & 1ull;
+            }
+            const float ey = fmaf(cur, p.pitch_y, p.xdc_min_y);
+            const MinMax dyr = dist_range(ey, Y0, Y1);
+            int c_lo = 0, c_hi = p.C - 1;
+            if (aperture_bounded) {
+                if (dyr.lo >= rmax_ap) continue;
+                const float hw = sqrtf(rmax_ap * rmax_ap - dyr.lo * dyr.lo);
+                c_lo = max(0, (int)floorf((X0 - hw - p.xdc_min_x) / p.pitch_x) - 1);
+                c_hi = min(p.C - 1, (int)ceilf((X1 + hw - p.xdc_min_x) / p.pitch_x) + 1);
+                if (c_lo > c_hi) continue;
+            }
+            const float dy = ey - vy;
+            const float dy2 = dy * dy;
+            const float2* trow = iq + (size_t)t * p.C * rowlen;
+            float apo_y = 1.0f;
+            if constexpr (SINAPO) apo_y = sin_apo_to(ey, p.xdc_max_y, p.to_power);
+
+            for (int cb = c_lo; cb <= c_hi; cb += HCB) {
+                const int nC = min(HCB, c_hi - cb + 1);
+                __syncthreads();
+                {
+                    const int wi = tid >> 4, part = tid & 15;
+                    if (wi < nC) {
+                        const int c = cb + wi;
+                        const float ex = fmaf((float)c, p.pitch_x, p.xdc_min_x);
+                        const MinMax dxr = dist_range(ex, X0, X1);
+                        const float rmin = sqrtf(dxr.lo * dxr.lo + dyr.lo * dyr.lo + zr.lo * zr.lo);
+                        const float rmax = sqrtf(dxr.hi * dxr.hi + dyr.hi * dyr.hi + zr.hi * zr.hi);
+                        float ilo = fmaf(txd_lo + rmin + fz, p.spm, (float)p.delay);
+                        float ihi = fmaf(txd_hi + rmax + fz, p.spm, (float)p.delay);
+                        if (p.spm < 0.0f) { const float tt = ilo; ilo = ihi; ihi = tt; }
+                        ilo = fminf(fmaxf(ilo, lo_clamp), hi_clamp);
+                        ihi = fminf(fmaxf(ihi, lo_clamp), hi_clamp);
+                        const int wlo = max(0, (int)floorf(ilo) - 2);
+                        const int whi = min(p.S - 1, (int)floorf(ihi) + 3);
+                        const int len = whi - wlo + 1;
+                        const int mode = (len <= HW && len > 0) ? 0 : 1;
+                        if (part == 0) { wlo_s[wi] = wlo; wmode_s[wi] = mode; }
+                        if (mode == 0) {
+                            const float2* row = trow + (size_t)c * rowlen;
+                            for (int q = part; q < len; q += 16) win[wi * HW + q] = row[wlo + q];
+                        }
+                    }
+                }
+                __syncthreads();
+                for (int cc = 0; cc < nC; ++cc) {
+                    const int c = cb + cc;
+                    const float ex = fmaf((float)c, p.pitch_x, p.xdc_min_x);
+                    const float dx = ex - vx;
+                    const float rho2 = fmaf(dx, dx, dy2);
+                    const int wlo = wlo_s[cc];
+                    const int mode = wmode_s[cc];
+                    const float2* wrow = win + cc * HW;
+                    const float2* grow = trow + (size_t)c * rowlen;
+                    float rho = 0.0f, apo_c = 0.0f;
+                    if constexpr (SINAPO) apo_c = sin_apo_to(ex, p.xdc_max_x, p.to_power) * apo_y;
+                    else rho = sqrtf(rho2);
+#pragma unroll
+                    for (int j = 0; j < HT_Z; ++j) {
+                        float apo;
+                        if constexpr (SINAPO) {
+                            apo = apo_c;
+                        } else {
+                            const float a = rho * kz[j];
+                            if (!(a < HANN_A_THR * 1.001f) && p.fn_rx > 0.0f) continue;
+                            apo = hann_from_a(a);
+                        }
+                        if (apo > 0.1f && zvalid[j] && col_valid) {
+                            const float r = sqrtf(rho2 + z2[j]);
+                            float x = fmaf(r, p.spm, base[j]);
+                            x = fminf(fmaxf(x, lo_clamp), hi_clamp);
+                            float2 v;
+                            if (mode == 0) {
+                                const float fi = floorf(x);
+                                const float tt = x - fi;
+                                int li = (int)fi - wlo;
+                                if constexpr (CUBIC) {
+                                    li = min(max(li, 1), HW - 3);
+                                    v = cubic2(wrow[li - 1], wrow[li], wrow[li + 1], wrow[li + 2], tt);
+                                } else {
+                                    li = min(max(li, 0), HW - 2);
+                                    v = lerp2(wrow[li], wrow[li + 1], tt);
+                                }
+                            } else {
+                                v = interp_global<CUBIC>(grow, x, p.S);
+                            }
+                            const float sa = neg ? -apo : apo;
+                            acc[j].x = fmaf(sa, v.x, acc[j].x);
+                            acc[j].y = fmaf(sa, v.y, acc[j].y);
+                            if constexpr (CF) inc[j] = fmaf(v.x, v.x, fmaf(v.y, v.y, inc[j]));
+                        }
+                    }
+                }
+            }
+        }
+    }
+    if (!col_valid) return;
+#pragma unroll
+    for (int j = 0; j < HT_Z; ++j) {
+        const int iz = bz + j;
+        if (iz >= p.Nz) break;
+        float2 a = acc[j];
+        if constexpr (CF) {
+            float cf = powf((a.x * a.x + a.y * a.y) / inc[j], p.cw);
+            if (isnan(cf)) cf = 0.0f;
+            a.x *= cf;
+            a.y *= cf;
+        }
+        vol[((size_t)iz * p.Ny + iy) * p.Nx + ix] = a;
+    }
+}
+
+// FORCES: one thread per voxel (the volume is y-invariant in the reference's
+// arithmetic; y only selects the output row).
+template <bool CUBIC>
+__global__ __launch_bounds__(256) void forces_kernel(const float2* __restrict__ iq, const DasParams p,
+                                                     float2* __restrict__ vol)
+{
+    const long long nvox = (long long)p.Nx * p.Ny * p.Nz;
+    const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nvox) return;
+    const int ix = (int)(v % p.Nx);
+    const int iz = (int)(v / ((long long)p.Nx * p.Ny));
+    const float vx = p.min_x + (float)ix * p.res_x;
+    const float vz = p.min_z + (float)iz * p.res_z;
+    const int G = p.readi_groups > 1 ? p.readi_groups : 1;
+    float2 acc = make_float2(0.0f, 0.0f);
+    float inc = 0.0f;
+    const float vz2 = vz * vz;
+    for (int c = 0; c < p.C; ++c) {
+        const float ec = fmaf((float)c, p.pitch_x, p.xdc_min_x);
+        const float rx = ec - vx;
+        const float rxd = sqrtf(fmaf(rx, rx, vz2));
+        float rx_apo;
+        if (p.apo_type == RX_HANN || p.apo_type == TX_TO_SIN) rx_apo = hann_from_a(p.fn_rx * (fabsf(rx) / vz) * 0.5f);
+        else rx_apo = sin_apo_to(ec, p.xdc_max_x, p.to_power);
+        if (!(rx_apo > 0.0f)) continue;
+        for (int g = 0; g < G; ++g) {
+            const bool neg = G > 1 && ((p.readi_bits >> g) & 1ull);
+            for (int t = 0; t < p.T; ++t) {
+                const float et = fmaf((float)(g * p.T + t), p.pitch_x, p.xdc_min_x);
+                const float tx = vx - et;
+                float tx_apo;
+                if (p.apo_type == BOTH_TO_SIN || p.apo_type == TX_TO_SIN) tx_apo = sin_apo_to(et, p.xdc_max_x, p.to_power);
+                else tx_apo = hann_from_a(p.fn_tx * (fabsf(tx) / vz) * 0.5f);
+                float apo = rx_apo * tx_apo;
+                float x = fmaf(sqrtf(fmaf(tx, tx, vz2)) + rxd, p.spm, (float)p.delay);
+                x = fminf(fmaxf(x, 1.0f), (float)p.S - 2.0f);
+                const float2 val = interp_global<CUBIC>(iq + ((size_t)t * p.C + c) * p.S, x, p.S);
+                if (neg) apo = -apo;
+                acc.x = fmaf(apo, val.x, acc.x);
+                acc.y = fmaf(apo, val.y, acc.y);
+                inc = fmaf(val.x, val.x, fmaf(val.y, val.y, inc));
+            }
+        }
+    }
+    float cf = powf((acc.x * acc.x + acc.y * acc.y) / inc, p.cw);
+    if (isnan(cf)) cf = 0.0f;
+    vol[v] = make_float2(acc.x * cf, acc.y * cf);
+}
+
+bool launch_das(const void* iq, int iq_kind, float iq_scale, const DasParams& p, const DasOptions& o, float2* vol,
+                hipStream_t st)
+{
+    (void)iq_scale;
+    if (iq_kind != 0) CUT_FAIL("DAS: staged IQ formats are not enabled in this build");
+    const float2* d = (const float2*)iq;
+    if (p.Nx <= 0 || p.Ny <= 0 || p.Nz <= 0) CUT_FAIL("DAS: empty output grid");
+    if (p.sequence == HERCULES) {
+        if (p.focus_type != 2 && p.focus_type != 4) CUT_FAIL("Invalid focus for HERCULES.");
+        dim3 grid((p.Nx + HT_X - 1) / HT_X, (p.Ny + HT_Y - 1) / HT_Y, (p.Nz + HT_Z - 1) / HT_Z);
+        const bool focus = p.focus_type == 4;
+        const bool sinapo = p.apo_type == RX_TO_SIN;
+        const bool cf = o.cf_hercules != 0;
+#define CUT_HERC(CU, FO, CFV, SA) hercules_kernel<CU, FO, CFV, SA><<<grid, 256, 0, st>>>(d, p, vol)
+#define CUT_HERC_SA(CU, FO, CFV) do { if (sinapo) CUT_HERC(CU, FO, CFV, true); else CUT_HERC(CU, FO, CFV, false); } while (0)
+#define CUT_HERC_CF(CU, FO) do { if (cf) CUT_HERC_SA(CU, FO, true); else CUT_HERC_SA(CU, FO, false); } while (0)
+#define CUT_HERC_FO(CU) do { if (focus) CUT_HERC_CF(CU, true); else CUT_HERC_CF(CU, false); } while (0)
+        if (o.cubic) CUT_HERC_FO(true); else CUT_HERC_FO(false);
+#undef CUT_HERC_FO
+#undef CUT_HERC_CF
+#undef CUT_HERC_SA
+#undef CUT_HERC
+    } else if (p.sequence == FORCES) {
+        if (p.readi_groups > 1 && p.encoding != HADAMARD) CUT_FAIL("FORCES only supports Hadamard encoding at this time.");
+        const long long nvox = (long long)p.Nx * p.Ny * p.Nz;
+        dim3 grid((unsigned)((nvox + 255) / 256));
+        if (o.cubic) forces_kernel<true><<<grid, 256, 0, st>>>(d, p, vol);
+        else forces_kernel<false><<<grid, 256, 0, st>>>(d, p, vol);
+    } else {
+        CUT_FAIL("Beamformer: Unsupported sequence ID " << p.sequence);
+    }
+    CUT_HIP_CHECK(hipGetLastError());
+    return true;
+}
+
+}  // namespace cut

@@ -1,0 +1,331 @@
+// Hilbert transform with optional match filter, one row per workgroup slice.
+//
+// Semantics: cuda_toolkit/src/rf_processing/rf_ffts/hilbert_handler.cu:132-155
+// and hilbert_kernels.cuh:17-40.  Per row x of length S:
+//   X = DFT_S(x);  Y[k] = (X[k] * w_k) * F[k] for k <= S/2, 0 above;
+//   w_0 = 0.75/S, w_k = 1.5/S (k >= 1, the Nyquist bin included);
+//   y = unnormalised inverse DFT_S(Y).
+//
+// Implementation: a Stockham auto-sort FFT held in LDS.  Each thread owns 16
+// complex values per pass (radix-16 passes, one radix-2/4/8 pass when log2 S
+// is not a multiple of 4), so S = 4096 takes three passes forward and three
+// inverse.  The first forward pass reads the input row straight from HBM
+// (coalesced; optionally converting raw int16/f32 through the channel map, so
+// convert+Hilbert is one kernel when no decode is needed), the spectral weight
+// and filter are applied as the last forward pass stores, and the last inverse
+// pass writes the complex row straight to HBM.  LDS rows are padded by one
+// complex per 16 so the radix-16 transposes are free of bank conflicts.
+// Twiddles come from a table generated in FP64.
+#include "common.hpp"
+
+namespace cut {
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <bool INV> __device__ __forceinline__ float2 cmul_tw(float2 a, float2 w)
+{
+    if constexpr (INV) w.y = -w.y;
+    return cmul(a, w);
+}
+// multiply by -i (forward) or +i (inverse)
+template <bool INV> __device__ __forceinline__ float2 mul_mi(float2 a)
+{
+    if constexpr (INV) return make_float2(-a.y, a.x);
+    else return make_float2(a.y, -a.x);
+}
+
+template <bool INV> __device__ __forceinline__ void dft2(float2& a, float2& b)
+{
+    float2 t = a;
+    a = cadd(t, b);
+    b = csub(t, b);
+}
+
+template <bool INV> __device__ __forceinline__ void dft4(float2& a0, float2& a1, float2& a2, float2& a3)
+{
+    float2 s02 = cadd(a0, a2), d02 = csub(a0, a2);
+    float2 s13 = cadd(a1, a3), d13 = mul_mi<INV>(csub(a1, a3));
+    a0 = cadd(s02, s13);
+    a2 = csub(s02, s13);
+    a1 = cadd(d02, d13);
+    a3 = csub(d02, d13);
+}
+
+// Exact constants of the 8th and 16th roots of unity.
+#define CUT_R2 0.70710678118654752440f
+#define CUT_C1 0.92387953251128675613f
+#define CUT_S1 0.38268343236508977173f
+
+template <bool INV> __device__ __forceinline__ float2 w16(float2 a, int e)
+{
+    // a * exp(-+ 2 pi i e / 16) for the exponents used by the 4x4 split
+    float2 w;
+    switch (e) {
+        case 1: w = make_float2(CUT_C1, -CUT_S1); break;
+        case 2: w = make_float2(CUT_R2, -CUT_R2); break;
+        case 3: w = make_float2(CUT_S1, -CUT_C1); break;
+        case 4: w = make_float2(0.0f, -1.0f); break;
+        case 6: w = make_float2(-CUT_R2, -CUT_R2); break;
+        case 9: w = make_float2(-CUT_C1, CUT_S1); break;
+        default: return a;
+    }
+    return cmul_tw<INV>(a, w);
+}
+
+template <int R, bool INV> __device__ __forceinline__ void dft(float2* u)
+{
+    if constexpr (R == 2) {
+        dft2<INV>(u[0], u[1]);
+    } else if constexpr (R == 4) {
+        dft4<INV>(u[0], u[1], u[2], u[3]);
+    } else if constexpr (R == 8) {
+        // n = g + 2h, k = q + 4m
+        float2 v0[4] = {u[0], u[2], u[4], u[6]};
+        float2 v1[4] = {u[1], u[3], u[5], u[7]};
+        dft4<INV>(v0[0], v0[1], v0[2], v0[3]);
+        dft4<INV>(v1[0], v1[1], v1[2], v1[3]);
+        // twiddle W8^(q) on the g = 1 branch
+        v1[1] = cmul_tw<INV>(v1[1], make_float2(CUT_R2, -CUT_R2));
+        v1[2] = mul_mi<INV>(v1[2]);
+        v1[3] = cmul_tw<INV>(v1[3], make_float2(-CUT_R2, -CUT_R2));
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            u[q] = cadd(v0[q], v1[q]);
+            u[q + 4] = csub(v0[q], v1[q]);
+        }
+    } else if constexpr (R == 16) {
+        // n = g + 4h, k = q + 4m: inner DFT4 over h, twiddle W16^(g q), outer DFT4 over g
+        float2 v[4][4];
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            v[g][0] = u[g]; v[g][1] = u[g + 4]; v[g][2] = u[g + 8]; v[g][3] = u[g + 12];
+            dft4<INV>(v[g][0], v[g][1], v[g][2], v[g][3]);
+        }
+#pragma unroll
+        for (int g = 1; g < 4; ++g)
+#pragma unroll
+            for (int q = 1; q < 4; ++q) v[g][q] = w16<INV>(v[g][q], g * q);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            float2 a0 = v[0][q], a1 = v[1][q], a2 = v[2][q], a3 = v[3][q];
+            dft4<INV>(a0, a1, a2, a3);
+            u[q] = a0; u[q + 4] = a1; u[q + 8] = a2; u[q + 12] = a3;
+        }
+    }
+}
+
+__device__ __forceinline__ int lds_idx(int i) { return i + (i >> 4); }
+
+struct HilbertDev {
+    const void* ptr;
+    int stride;
+    const int16_t* map;
+    size_t R0;
+    int C;
+};
+
+template <int KIND>
+__device__ __forceinline__ float load_in(const HilbertDev& in, int row, int S, int idx)
+{
+    if constexpr (KIND == 0) {
+        return ((const float*)in.ptr)[((size_t)row * S + idx) * in.stride];
+    } else {
+        const int t = row / in.C, c = row - t * in.C;
+        const size_t off = (size_t)(uint16_t)in.map[c] * in.R0 + (size_t)t * S + idx;
+        if constexpr (KIND == 1) return (float)((const int16_t*)in.ptr)[off];
+        else return ((const float*)in.ptr)[off];
+    }
+}
+
+// One Stockham pass over an S-point row held by P = S/16 threads.  The thread
+// holds 16 values: 16/R butterflies of radix R.  Load phase -> (barrier) ->
+// twiddle + DFT + store phase.
+template <int S, int R, bool INV, typename LoadF, typename StoreF>
+__device__ __forceinline__ void stockham_pass(int tid, int p, const float2* __restrict__ tw, LoadF load,
+                                              StoreF store, bool sync_between)
+{
+    constexpr int P = S / 16;
+    constexpr int NB = S / R;
+    constexpr int PER = 16 / R;
+    float2 u[16];
+#pragma unroll
+    for (int m = 0; m < PER; ++m) {
+        const int i = tid + m * P;
+#pragma unroll
+        for (int r = 0; r < R; ++r) u[m * R + r] = load(i + r * NB);
+    }
+    if (sync_between) __syncthreads();
+#pragma unroll
+    for (int m = 0; m < PER; ++m) {
+        const int i = tid + m * P;
+        const int k = i & (p - 1);
+        if (p > 1) {
+            const int step = S / (p * R);
+#pragma unroll
+            for (int r = 1; r < R; ++r) u[m * R + r] = cmul_tw<INV>(u[m * R + r], tw[(r * k * step) & (S - 1)]);
+        }
+        dft<R, INV>(u + m * R);
+        const int base = (i - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) store(base + r * p, u[m * R + r]);
+    }
+}
+
+template <int LOGS, int KIND, bool FILTER>
+__global__ __launch_bounds__(256) void hilbert_kernel(HilbertDev in, int rows, const float2* __restrict__ tw,
+                                                      const float2* __restrict__ F, float2* __restrict__ out)
+{
+    constexpr int S = 1 << LOGS;
+    constexpr int P = S / 16;
+    constexpr int RPW = P >= 256 ? 1 : 256 / P;     // rows per workgroup
+    constexpr int NP16 = LOGS / 4;                   // radix-16 passes
+    constexpr int RT = 1 << (LOGS % 4);             // trailing radix (1 = none)
+    constexpr int PADDED = S + S / 16;
+    __shared__ float2 buf[RPW * PADDED];
+
+    const int tid = threadIdx.x % P;
+    const int rsub = threadIdx.x / P;
+    const int row = blockIdx.x * RPW + rsub;
+    const bool active = row < rows;
+    float2* lds = buf + rsub * PADDED;
+    const int rrow = active ? row : 0;
+
+    const float scale = 1.5f / (float)S;
+    auto load_global = [&](int idx) { return make_float2(active ? load_in<KIND>(in, rrow, S, idx) : 0.0f, 0.0f); };
+    auto load_lds = [&](int idx) { return lds[lds_idx(idx)]; };
+    auto store_lds = [&](int idx, float2 v) { lds[lds_idx(idx)] = v; };
+    auto store_spec = [&](int k, float2 v) {
+        if (k > S / 2) {
+            v = make_float2(0.0f, 0.0f);
+        } else {
+            const float w = k == 0 ? scale * 0.5f : scale;
+            v = make_float2(v.x * w, v.y * w);
+            if constexpr (FILTER) v = cmul(v, F[k]);
+        }
+        lds[lds_idx(k)] = v;
+    };
+    float2* orow = out + (size_t)rrow * S;
+    auto store_out = [&](int idx, float2 v) { if (active) orow[idx] = v; };
+
+    // forward
+    int p = 1;
+#pragma unroll
+    for (int q = 0; q < NP16; ++q) {
+        const bool last = (q == NP16 - 1) && RT == 1;
+        if (q == 0) {
+            if (last) stockham_pass<S, 16, false>(tid, p, tw, load_global, store_spec, false);
+            else stockham_pass<S, 16, false>(tid, p, tw, load_global, store_lds, false);
+        } else {
+            if (last) stockham_pass<S, 16, false>(tid, p, tw, load_lds, store_spec, true);
+            else stockham_pass<S, 16, false>(tid, p, tw, load_lds, store_lds, true);
+        }
+        __syncthreads();
+        p *= 16;
+    }
+    if constexpr (RT > 1) {
+        if (NP16 == 0) stockham_pass<S, RT, false>(tid, p, tw, load_global, store_spec, false);
+        else stockham_pass<S, RT, false>(tid, p, tw, load_lds, store_spec, true);
+        __syncthreads();
+    }
+    // inverse
+    p = 1;
+#pragma unroll
+    for (int q = 0; q < NP16; ++q) {
+        const bool last = (q == NP16 - 1) && RT == 1;
+        if (last) stockham_pass<S, 16, true>(tid, p, tw, load_lds, store_out, true);
+        else {
+            stockham_pass<S, 16, true>(tid, p, tw, load_lds, store_lds, true);
+            __syncthreads();
+        }
+        p *= 16;
+    }
+    if constexpr (RT > 1) stockham_pass<S, RT, true>(tid, p, tw, load_lds, store_out, true);
+}
+
+// Twiddle table tw[m] = exp(-2 pi i m / S), generated in FP64.
+__global__ void twiddle_kernel(float2* tw, int S)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= S) return;
+    double sn, cs;
+    sincospi(2.0 * (double)m / (double)S, &sn, &cs);
+    tw[m] = make_float2((float)cs, (float)-sn);
+}
+
+// Match-filter spectrum F[k] = sum_{n<L} h[n] exp(-2 pi i k n / S) in FP64
+// (hilbert_handler.cu:55-78 takes the forward DFT of the zero-padded taps).
+__global__ void filter_spectrum_kernel(const float* h, int L, int S, float2* F)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= S) return;
+    double re = 0.0, im = 0.0;
+    for (int n = 0; n < L; ++n) {
+        const long long e = ((long long)k * n) % S;
+        double sn, cs;
+        sincospi(2.0 * (double)e / (double)S, &sn, &cs);
+        re += (double)h[n] * cs;
+        im -= (double)h[n] * sn;
+    }
+    F[k] = make_float2((float)re, (float)im);
+}
+
+bool launch_twiddles(float2* d_tw, int S, hipStream_t st)
+{
+    twiddle_kernel<<<(S + 255) / 256, 256, 0, st>>>(d_tw, S);
+    CUT_HIP_CHECK(hipGetLastError());
+    return true;
+}
+
+bool launch_filter_spectrum(const float* d_h, int L, int S, float2* d_F, hipStream_t st)
+{
+    filter_spectrum_kernel<<<(S + 255) / 256, 256, 0, st>>>(d_h, L, S, d_F);
+    CUT_HIP_CHECK(hipGetLastError());
+    return true;
+}
+
+bool hilbert_supported(int S) { return S >= 16 && S <= 4096 && (S & (S - 1)) == 0; }
+
+template <int LOGS>
+static bool launch_hilbert_s(const HilbertInput& in, int rows, const float2* tw, const float2* F, float2* out,
+                             hipStream_t st)
+{
+    constexpr int S = 1 << LOGS;
+    constexpr int P = S / 16;
+    constexpr int RPW = P >= 256 ? 1 : 256 / P;
+    HilbertDev d{in.ptr, in.stride, in.map, in.R0, in.C};
+    dim3 grid((rows + RPW - 1) / RPW);
+    dim3 block(P * RPW);
+#define CUT_HIL(K, FL) hilbert_kernel<LOGS, K, FL><<<grid, block, 0, st>>>(d, rows, tw, F, out)
+    if (F) {
+        if (in.kind == 0) CUT_HIL(0, true); else if (in.kind == 1) CUT_HIL(1, true); else CUT_HIL(2, true);
+    } else {
+        if (in.kind == 0) CUT_HIL(0, false); else if (in.kind == 1) CUT_HIL(1, false); else CUT_HIL(2, false);
+    }
+#undef CUT_HIL
+    CUT_HIP_CHECK(hipGetLastError());
+    return true;
+}
+
+bool launch_hilbert(const HilbertInput& in, int rows, int S, const float2* d_tw, const float2* d_F, float2* out,
+                    hipStream_t st)
+{
+    if (rows <= 0) return true;
+    switch (S) {
+        case 16: return launch_hilbert_s<4>(in, rows, d_tw, d_F, out, st);
+        case 32: return launch_hilbert_s<5>(in, rows, d_tw, d_F, out, st);
+        case 64: return launch_hilbert_s<6>(in, rows, d_tw, d_F, out, st);
+        case 128: return launch_hilbert_s<7>(in, rows, d_tw, d_F, out, st);
+        case 256: return launch_hilbert_s<8>(in, rows, d_tw, d_F, out, st);
+        case 512: return launch_hilbert_s<9>(in, rows, d_tw, d_F, out, st);
+        case 1024: return launch_hilbert_s<10>(in, rows, d_tw, d_F, out, st);
+        case 2048: return launch_hilbert_s<11>(in, rows, d_tw, d_F, out, st);
+        case 4096: return launch_hilbert_s<12>(in, rows, d_tw, d_F, out, st);
+        default: CUT_FAIL("Hilbert: sample count " << S << " not supported (powers of two 16..4096)");
+    }
+}
+
+}  // namespace cut

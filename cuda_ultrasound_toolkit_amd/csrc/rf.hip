@@ -1,0 +1,271 @@
+// RF conversion and Hadamard decode kernels for gfx950.
+//
+// convert: cuda_toolkit/src/rf_processing/data_conversion/data_conversion_kernels.cuh:25-41
+// decode:  cuda_toolkit/src/rf_processing/hadamard/hadamard_decoder.cpp:49-78 (cuBLAS SGEMM)
+//
+// The decode is a GEMM per channel c: Out_c[j][s] = scale * sum_k H[j][k] X_c[k][s]
+// with X_c[k][s] = raw[map[c]][k*S + s].  A workgroup owns one channel and 64
+// consecutive samples, all output rows.  The int16 tile is read once from HBM
+// (16-byte loads), split into bytes and transposed into LDS so that each lane
+// holds 16 consecutive k for one sample column, the layout MFMA wants for its
+// B operand.  Two MFMA paths:
+//   mode 1 (int8): x = 256*hi + (lo_u - 128) + 128 with hi, lo_u-128 in int8;
+//                  sum_k H x = 256*(H.hi) + H.(lo-128) + 128*rowsum(H), all in
+//                  int32, exact for every int16 input and T <= 1024.
+//   mode 0 (fp32): v_mfma_f32_32x32x2_f32 on the converted floats; exact for
+//                  int16 input and power-of-two T (every partial sum is an
+//                  integer below 2^24), so both modes agree bit for bit.
+#include "common.hpp"
+
+namespace cut {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+typedef float v16f __attribute__((ext_vector_type(16)));
+
+// ---------------------------------------------------------------------------
+// convert (no decode)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void convert_kernel(const T* __restrict__ raw, const int16_t* __restrict__ map,
+                                                      size_t R0, int C, int S, float* __restrict__ out,
+                                                      int out_stride)
+{
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    const int c = blockIdx.y;
+    const int t = blockIdx.z;
+    if (s >= S) return;
+    const T* src = raw + (size_t)(uint16_t)map[c] * R0 + (size_t)t * S;
+    out[(((size_t)t * C + c) * S + s) * out_stride] = static_cast<float>(src[s]);
+}
+
+bool launch_convert(const void* raw, int type, const int16_t* d_map, size_t R0, int T, int C, int S,
+                    float* out, int out_stride, hipStream_t st)
+{
+    dim3 grid((S + 255) / 256, C, T);
+    if (type == TYPE_I16)
+        convert_kernel<int16_t><<<grid, 256, 0, st>>>((const int16_t*)raw, d_map, R0, C, S, out, out_stride);
+    else if (type == TYPE_F32)
+        convert_kernel<float><<<grid, 256, 0, st>>>((const float*)raw, d_map, R0, C, S, out, out_stride);
+    else
+        CUT_FAIL("data converter: unsupported input data type " << type);
+    CUT_HIP_CHECK(hipGetLastError());
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// decode
+// ---------------------------------------------------------------------------
+constexpr int DEC_NT = 64;      // samples per workgroup
+constexpr int DEC_KC = 128;     // k rows staged per chunk
+constexpr int DEC_ROWB = 144;   // LDS bytes per sample column of a byte tile (16-byte padded)
+constexpr int DEC_ROWF = 65;    // LDS floats per k row of the fp32 tile (bank padded)
+
+// Load 8 consecutive samples (k row, sample s) as int32 values.
+template <typename T, bool VEC>
+__device__ __forceinline__ void load8(const T* __restrict__ row, int s, int S, int v[8])
+{
+    if constexpr (VEC && sizeof(T) == 2) {
+        if (s + 8 <= S) {
+            v4i q = *(const v4i*)(row + s);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                v[2 * e] = (int)(int16_t)(q[e] & 0xffff);
+                v[2 * e + 1] = (int)(int16_t)((unsigned)q[e] >> 16);
+            }
+            return;
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = (s + e < S) ? (int)row[s + e] : 0;
+}
+
+template <bool VEC, int OUT_STRIDE>
+__global__ __launch_bounds__(256) void decode_i8_kernel(const int16_t* __restrict__ raw,
+                                                        const int16_t* __restrict__ map, size_t R0, int T,
+                                                        int C, int S, int Kp, const int8_t* __restrict__ h8,
+                                                        const int* __restrict__ rowsum, float scale,
+                                                        float* __restrict__ out)
+{
+    __shared__ __attribute__((aligned(16))) int8_t bhi[DEC_NT * DEC_ROWB];
+    __shared__ __attribute__((aligned(16))) int8_t blo[DEC_NT * DEC_ROWB];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int s0 = blockIdx.x * DEC_NT;
+    const int c = blockIdx.y;
+    const int mbase = blockIdx.z * 128 + wave * 32;
+    const bool mvalid = mbase < Kp;
+    const int16_t* src = raw + (size_t)(uint16_t)map[c] * R0;
+
+    v16i acc_hi[2], acc_lo[2];
+#pragma unroll
+    for (int n = 0; n < 2; ++n)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) { acc_hi[n][r] = 0; acc_lo[n][r] = 0; }
+
+    const int kq = tid >> 3, sg = tid & 7;
+    for (int k0 = 0; k0 < Kp; k0 += DEC_KC) {
+        const int kc = min(DEC_KC, Kp - k0);
+        __syncthreads();
+        if (kq * 4 < kc) {
+            int v[4][8];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int k = k0 + kq * 4 + j;
+                if (k < T) load8<int16_t, VEC>(src + (size_t)k * S, s0 + sg * 8, S, v[j]);
+                else {
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) v[j][e] = 0;
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                unsigned hi = 0, lo = 0;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const unsigned x = (unsigned)v[j][e];
+                    hi |= ((x >> 8) & 0xffu) << (8 * j);
+                    lo |= ((x & 0xffu) ^ 0x80u) << (8 * j);
+                }
+                const int off = (sg * 8 + e) * DEC_ROWB + kq * 4;
+                *(unsigned*)&bhi[off] = hi;
+                *(unsigned*)&blo[off] = lo;
+            }
+        }
+        __syncthreads();
+        if (mvalid) {
+            const int8_t* hrow = h8 + (size_t)(mbase + (lane & 31)) * Kp + k0 + (lane >> 5) * 16;
+            for (int kk = 0; kk < kc; kk += 32) {
+                const v4i a = *(const v4i*)(hrow + kk);
+#pragma unroll
+                for (int nt = 0; nt < 2; ++nt) {
+                    const int off = (nt * 32 + (lane & 31)) * DEC_ROWB + kk + (lane >> 5) * 16;
+                    const v4i bh = *(const v4i*)&bhi[off];
+                    const v4i bl = *(const v4i*)&blo[off];
+                    acc_hi[nt] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, bh, acc_hi[nt], 0, 0, 0);
+                    acc_lo[nt] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, bl, acc_lo[nt], 0, 0, 0);
+                }
+            }
+        }
+    }
+    if (!mvalid) return;
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt) {
+        const int s = s0 + nt * 32 + (lane & 31);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int m = mbase + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            if (m < T && s < S) {
+                const int total = 256 * acc_hi[nt][r] + acc_lo[nt][r] + 128 * rowsum[m];
+                out[(((size_t)m * C + c) * S + s) * OUT_STRIDE] = (float)total * scale;
+            }
+        }
+    }
+}
+
+template <typename TIN, bool VEC, int OUT_STRIDE>
+__global__ __launch_bounds__(256) void decode_f32_kernel(const TIN* __restrict__ raw,
+                                                         const int16_t* __restrict__ map, size_t R0, int T,
+                                                         int C, int S, int Kp, const float* __restrict__ hf,
+                                                         float scale, float* __restrict__ out)
+{
+    __shared__ float bt[DEC_KC * DEC_ROWF];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int s0 = blockIdx.x * DEC_NT;
+    const int c = blockIdx.y;
+    const int mbase = blockIdx.z * 128 + wave * 32;
+    const bool mvalid = mbase < Kp;
+    const TIN* src = raw + (size_t)(uint16_t)map[c] * R0;
+
+    v16f acc[2];
+#pragma unroll
+    for (int n = 0; n < 2; ++n)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[n][r] = 0.0f;
+
+    const int kq = tid >> 3, sg = tid & 7;
+    for (int k0 = 0; k0 < Kp; k0 += DEC_KC) {
+        const int kc = min(DEC_KC, Kp - k0);
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int kl = kq * 4 + j;
+            if (kl >= kc) break;
+            const int k = k0 + kl;
+            float* dst = bt + kl * DEC_ROWF + sg * 8;
+            if (k < T) {
+                if constexpr (sizeof(TIN) == 2) {
+                    int v[8];
+                    load8<int16_t, VEC>((const int16_t*)src + (size_t)k * S, s0 + sg * 8, S, v);
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) dst[e] = (float)v[e];
+                } else {
+                    const float* row = (const float*)src + (size_t)k * S;
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) {
+                        const int s = s0 + sg * 8 + e;
+                        dst[e] = s < S ? row[s] : 0.0f;
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) dst[e] = 0.0f;
+            }
+        }
+        __syncthreads();
+        if (mvalid) {
+            const float* hrow = hf + (size_t)(mbase + (lane & 31)) * Kp + k0 + (lane >> 5);
+            for (int kk = 0; kk < kc; kk += 2) {
+                const float a = hrow[kk];
+                const float* brow = bt + (kk + (lane >> 5)) * DEC_ROWF + (lane & 31);
+#pragma unroll
+                for (int nt = 0; nt < 2; ++nt)
+                    acc[nt] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, brow[nt * 32], acc[nt], 0, 0, 0);
+            }
+        }
+    }
+    if (!mvalid) return;
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt) {
+        const int s = s0 + nt * 32 + (lane & 31);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int m = mbase + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            if (m < T && s < S) out[(((size_t)m * C + c) * S + s) * OUT_STRIDE] = acc[nt][r] * scale;
+        }
+    }
+}
+
+bool launch_decode(const void* raw, int type, const int16_t* d_map, size_t R0, int T, int C, int S,
+                   const int8_t* d_h8, const float* d_hf, const int* d_rowsum, float scale, float* out,
+                   int out_stride, int mode, hipStream_t st)
+{
+    if (out_stride != 1 && out_stride != 2) CUT_FAIL("decode: output stride must be 1 or 2");
+    const int Kp = (T + 31) / 32 * 32;
+    dim3 grid((S + DEC_NT - 1) / DEC_NT, C, (Kp + 127) / 128);
+    const bool vec = ((uintptr_t)raw % 16 == 0) && (R0 % 8 == 0) && (S % 8 == 0);
+    if (type == TYPE_I16 && mode == 1) {
+        const int16_t* r = (const int16_t*)raw;
+#define CUT_DEC_I8(V, O) decode_i8_kernel<V, O><<<grid, 256, 0, st>>>(r, d_map, R0, T, C, S, Kp, d_h8, d_rowsum, scale, out)
+        if (vec) { if (out_stride == 1) CUT_DEC_I8(true, 1); else CUT_DEC_I8(true, 2); }
+        else { if (out_stride == 1) CUT_DEC_I8(false, 1); else CUT_DEC_I8(false, 2); }
+#undef CUT_DEC_I8
+    } else if (type == TYPE_I16) {
+        const int16_t* r = (const int16_t*)raw;
+#define CUT_DEC_F(V, O) decode_f32_kernel<int16_t, V, O><<<grid, 256, 0, st>>>(r, d_map, R0, T, C, S, Kp, d_hf, scale, out)
+        if (vec) { if (out_stride == 1) CUT_DEC_F(true, 1); else CUT_DEC_F(true, 2); }
+        else { if (out_stride == 1) CUT_DEC_F(false, 1); else CUT_DEC_F(false, 2); }
+#undef CUT_DEC_F
+    } else if (type == TYPE_F32) {
+        const float* r = (const float*)raw;
+        if (out_stride == 1)
+            decode_f32_kernel<float, false, 1><<<grid, 256, 0, st>>>(r, d_map, R0, T, C, S, Kp, d_hf, scale, out);
+        else
+            decode_f32_kernel<float, false, 2><<<grid, 256, 0, st>>>(r, d_map, R0, T, C, S, Kp, d_hf, scale, out);
+    } else {
+        CUT_FAIL("data converter: unsupported input data type " << type);
+    }
+    CUT_HIP_CHECK(hipGetLastError());
+    return true;
+}
+
+}  // namespace cut

@@ -1,0 +1,100 @@
+/*
+ * Extensions of the toolkit's C ABI (no counterpart in the reference).
+ *
+ * Plain C, plain pointers and sizes.  "d_" pointers are device (HBM)
+ * pointers; these entry points enqueue on the library stream and return
+ * without waiting unless stated, so a caller can time or overlap them.  They
+ * exist for callers whose data is already resident in HBM (benchmarks,
+ * pipelines that produce RF on the GPU), for per-stage testing, and to select
+ * the build's precision/interpolation variants.  All return false and print a
+ * message on std::cerr on failure, like the reference API.
+ */
+#ifndef CUT_EXT_H
+#define CUT_EXT_H
+
+#include <stddef.h>
+#include <stdint.h>
+#ifndef __cplusplus
+#include <stdbool.h>
+#endif
+#include "cuda_beamformer_parameters.h"
+
+#define CUT_EXT_EXPORT __attribute__((visibility("default")))
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- runtime ---------------------------------------------------------- */
+/* Use the caller's hipStream_t (NULL restores the library's own stream). */
+CUT_EXT_EXPORT void cut_set_stream(void* hip_stream);
+CUT_EXT_EXPORT bool cut_synchronize(void);
+CUT_EXT_EXPORT const char* cut_version(void);
+
+/* ---- variants ----------------------------------------------------------- */
+/* decode: 0 auto (int8 MFMA for int16 input, FP32 MFMA GEMM otherwise),
+ *         1 FP32 MFMA GEMM, 2 int8 MFMA (int16 input only). */
+CUT_EXT_EXPORT bool cut_set_decode_mode(int mode);
+/* DAS: cubic 1 = Catmull-Rom as the reference, 0 = linear;
+ * cf_hercules 1 = apply the coherence factor in HERCULES too (the reference
+ * computes it but does not apply it); iq_format 0 = complex f32. */
+CUT_EXT_EXPORT bool cut_set_das_options(int cubic, int cf_hercules, int iq_format);
+
+/* ---- whole path on device pointers (async) ------------------------------ */
+/* Same semantics as cuda_toolkit::beamform (cuda_toolkit/src/api/cuda_toolkit.cpp:109-199)
+ * with the raw frame and the volume already in HBM. */
+CUT_EXT_EXPORT bool cut_beamform_device(const void* d_raw, size_t raw_bytes, void* d_volume, size_t volume_bytes,
+                                        const CudaBeamformerParameters* bp);
+
+/* ---- stages on device pointers (async), one RF state of their own ------- */
+/* RfProcessor::init (cuda_toolkit/src/rf_processing/rf_processor.cpp:21-71):
+ * raw_dims = {samples*transmits + pad, raw channels}; dec_dims = {S, C, T}. */
+CUT_EXT_EXPORT bool cut_rf_init(const uint* raw_dims, const uint* dec_dims, int encoding);
+CUT_EXT_EXPORT bool cut_rf_set_channel_mapping(const int16_t* map, uint count);
+CUT_EXT_EXPORT bool cut_rf_set_match_filter(const float* taps, uint length);
+/* convert (+ decode when decode != 0) into d_out[(t*C+c)*S+s]*out_stride (floats);
+ * out_stride 2 writes the real parts of a complex buffer, as the reference does. */
+CUT_EXT_EXPORT bool cut_rf_convert_decode(const void* d_raw, int type, float* d_out, int out_stride, int decode);
+/* Hilbert (+ match filter) of rows read at d_in[(row*S+s)*in_stride] -> complex d_out. */
+CUT_EXT_EXPORT bool cut_rf_hilbert(const float* d_in, int in_stride, void* d_out);
+/* DAS of complex IQ [T][C][S] into a complex volume [Nz][Ny][Nx]
+ * (cuda_toolkit/src/beamformer/beamformer.cu:139-197). */
+CUT_EXT_EXPORT bool cut_das_device(const void* d_iq, void* d_volume, const CudaBeamformerParameters* bp);
+
+/* Number of (voxel, transmit, channel) taps whose apodization passes the
+ * sequence's threshold: the work one DAS launch does (host computation). */
+CUT_EXT_EXPORT bool cut_das_active_taps(const CudaBeamformerParameters* bp, unsigned long long* taps);
+
+/* ---- GL seam ----------------------------------------------------------- */
+/* Register plain device buffers in place of GL buffers so the C API of
+ * cuda_toolkit_ogl.h can run where no GL context exists (tests, headless
+ * servers).  rf_bytes[i] is the size of rf_ptrs[i]. */
+CUT_EXT_EXPORT bool cut_gl_register_device_buffers(void* const* rf_ptrs, const size_t* rf_bytes, uint count,
+                                                   void* raw_ptr, size_t raw_bytes);
+
+/* ---- synthetic acquisitions (test and benchmark inputs) ----------------- */
+typedef struct CutSynthSpec {
+    int geometry;          /* 0: linear array, FORCES delay model; 1: row-column array, HERCULES model */
+    int T, C, S;           /* transmits (encoding order), channels, samples */
+    int n_scatterers;
+    float pitch;           /* [m] element pitch (both axes) */
+    float c, fc, fs;       /* [m/s], [Hz], [Hz] */
+    float box_min[3], box_max[3];  /* scatterer box [m] */
+    float amp_min, amp_max;
+    float noise_frac;      /* noise std as a fraction of the peak single-element response */
+    float peak;            /* int16 full scale of the encoded data */
+    int encoding;          /* NONE, HADAMARD or WALSH */
+    int cycles;            /* pulse length in periods */
+    unsigned long long seed;
+    float shift[3];        /* rigid displacement applied to every scatterer [m] */
+    float decorrelation;   /* fraction of scatterer amplitude redrawn (0 = identical speckle) */
+    unsigned long long amp_seed; /* seed of the redrawn amplitudes */
+} CutSynthSpec;
+/* raw: int16 [C][T*S] (row r = raw channel r).  map (C entries, may be NULL)
+ * receives decoded channel -> raw row, a permutation drawn from the seed. */
+CUT_EXT_EXPORT bool cut_synth_rf(const CutSynthSpec* spec, int16_t* raw, int16_t* map);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CUT_EXT_H */

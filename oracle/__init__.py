@@ -1,0 +1,170 @@
+"""ctypes wrapper of the FP64 CPU oracle (oracle.c).
+
+TEST INFRASTRUCTURE ONLY: the checker for the HIP path.  Imported by tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg; never by the
+cuda_ultrasound_toolkit_amd package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+_lib = None
+
+DP = C.POINTER(C.c_double)
+
+
+class DasConsts(C.Structure):
+    _fields_ = [
+        ("S", C.c_int), ("C", C.c_int), ("T", C.c_int),
+        ("Nx", C.c_int), ("Ny", C.c_int), ("Nz", C.c_int),
+        ("min_x", C.c_float), ("min_y", C.c_float), ("min_z", C.c_float),
+        ("res_x", C.c_float), ("res_y", C.c_float), ("res_z", C.c_float),
+        ("xdc_min_x", C.c_float), ("xdc_min_y", C.c_float), ("xdc_max_x", C.c_float), ("xdc_max_y", C.c_float),
+        ("pitch_x", C.c_float), ("pitch_y", C.c_float),
+        ("spm", C.c_float),
+        ("delay", C.c_int),
+        ("focal_z", C.c_float),
+        ("focus_type", C.c_int),
+        ("fn_tx", C.c_float), ("fn_rx", C.c_float), ("cw", C.c_float),
+        ("readi_groups", C.c_int), ("readi_id", C.c_int), ("encoding", C.c_int),
+        ("apo_type", C.c_int), ("to_power", C.c_int),
+        ("sequence", C.c_int),
+    ]
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB.exists():
+        build()
+    lib = C.CDLL(str(LIB))
+    lib.or_hadamard.restype = C.c_int
+    lib.or_hadamard.argtypes = [C.c_int, C.c_int, DP]
+    lib.or_decode.argtypes = [DP, DP, C.c_int, C.c_size_t, DP]
+    lib.or_convert.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int16), C.c_size_t, C.c_int, C.c_int, C.c_int, DP]
+    lib.or_dft.argtypes = [DP, DP, C.c_int, C.c_int, DP, DP]
+    lib.or_fft.argtypes = [DP, DP, C.c_int, C.c_int]
+    lib.or_filter_spectrum.argtypes = [DP, C.c_int, C.c_int, DP]
+    lib.or_hilbert.argtypes = [DP, C.c_size_t, C.c_int, DP, C.c_int, DP]
+    lib.or_das_constants.restype = C.c_int
+    lib.or_das_constants.argtypes = [C.c_void_p, C.POINTER(DasConsts)]
+    lib.or_das.restype = C.c_int
+    lib.or_das.argtypes = [DP, C.POINTER(DasConsts), C.c_int, C.c_int, DP, C.POINTER(C.c_longlong)]
+    _lib = lib
+    return lib
+
+
+def _dp(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(DP)
+
+
+def hadamard(n: int, ordering: int) -> np.ndarray | None:
+    H = np.zeros((n, n), np.float64)
+    return H if load().or_hadamard(n, ordering, _dp(H)) == 0 else None
+
+
+def convert(raw: np.ndarray, dtype: int, cmap: np.ndarray, R0: int, T: int, Cc: int, S: int) -> np.ndarray:
+    out = np.zeros((T, Cc, S), np.float64)
+    m = np.ascontiguousarray(cmap, np.int16)
+    raw = np.ascontiguousarray(raw)
+    load().or_convert(raw.ctypes.data, dtype, m.ctypes.data_as(C.POINTER(C.c_int16)), R0, T, Cc, S, _dp(out))
+    return out
+
+
+def decode(conv: np.ndarray, H: np.ndarray) -> np.ndarray:
+    T = H.shape[0]
+    conv = np.ascontiguousarray(conv, np.float64)
+    cols = conv.size // T
+    out = np.zeros_like(conv)
+    load().or_decode(_dp(conv), _dp(np.ascontiguousarray(H)), T, cols, _dp(out))
+    return out
+
+
+def dft(x: np.ndarray, sign: int = -1) -> np.ndarray:
+    x = np.asarray(x, np.complex128)
+    re = np.ascontiguousarray(x.real)
+    im = np.ascontiguousarray(x.imag)
+    ore = np.zeros(len(x))
+    oim = np.zeros(len(x))
+    load().or_dft(_dp(re), _dp(im), len(x), sign, _dp(ore), _dp(oim))
+    return ore + 1j * oim
+
+
+def fft(x: np.ndarray, sign: int = -1) -> np.ndarray:
+    x = np.asarray(x, np.complex128)
+    re = np.ascontiguousarray(x.real.copy())
+    im = np.ascontiguousarray(x.imag.copy())
+    load().or_fft(_dp(re), _dp(im), len(x), sign)
+    return re + 1j * im
+
+
+def filter_spectrum(h: np.ndarray, S: int) -> np.ndarray:
+    h = np.ascontiguousarray(h, np.float64)
+    out = np.zeros(2 * S)
+    load().or_filter_spectrum(_dp(h), len(h), S, _dp(out))
+    return out[0::2] + 1j * out[1::2]
+
+
+def hilbert(x: np.ndarray, h: np.ndarray | None = None) -> np.ndarray:
+    """x: real rows [..., S] -> complex rows (reference Hilbert + match filter)."""
+    x = np.ascontiguousarray(x, np.float64)
+    S = x.shape[-1]
+    rows = x.size // S
+    out = np.zeros(2 * x.size)
+    if h is None or len(h) == 0:
+        load().or_hilbert(_dp(x), rows, S, None, 0, _dp(out))
+    else:
+        hh = np.ascontiguousarray(h, np.float64)
+        load().or_hilbert(_dp(x), rows, S, _dp(hh), len(hh), _dp(out))
+    return (out[0::2] + 1j * out[1::2]).reshape(x.shape)
+
+
+def das_constants(bp) -> DasConsts:
+    k = DasConsts()
+    load().or_das_constants(C.addressof(bp), C.byref(k))
+    return k
+
+
+def das(iq: np.ndarray, bp, cubic: bool = True, cf_hercules: bool = False):
+    """iq: complex [T][C][S] -> (complex volume [Nz][Ny][Nx], active tap count)."""
+    k = das_constants(bp)
+    iqd = np.ascontiguousarray(np.stack([iq.real, iq.imag], axis=-1), np.float64)
+    out = np.zeros(2 * k.Nx * k.Ny * k.Nz)
+    act = C.c_longlong(0)
+    rc = load().or_das(_dp(iqd), C.byref(k), 0 if cubic else 1, 1 if cf_hercules else 0, _dp(out), C.byref(act))
+    if rc != 0:
+        raise ValueError("oracle DAS rejected the parameters")
+    vol = (out[0::2] + 1j * out[1::2]).reshape(k.Nz, k.Ny, k.Nx)
+    return vol, act.value
+
+
+def pipeline(acq, cubic: bool = True, cf_hercules: bool = False, return_stages: bool = False):
+    """The whole reference chain on the CPU: convert -> decode -> Hilbert -> DAS."""
+    bp = acq.bp
+    S, Cc, T = bp.dec_data_dim[0], bp.dec_data_dim[1], bp.dec_data_dim[2]
+    R0 = bp.rf_raw_dim[0]
+    conv = convert(acq.raw, bp.data_type, np.array(bp.channel_mapping[:Cc], np.int16), R0, T, Cc, S)
+    if bp.decode:
+        H = hadamard(T, bp.decode)
+        dec = decode(conv, H)
+    else:
+        dec = conv
+    h = np.array(bp.rf_filter[: bp.filter_length], np.float64) if bp.filter_length else None
+    iq = hilbert(dec.reshape(T * Cc, S), h).reshape(T, Cc, S)
+    vol, act = das(iq, bp, cubic, cf_hercules)
+    if return_stages:
+        return dict(conv=conv, dec=dec, iq=iq, vol=vol, active=act)
+    return vol

@@ -1,0 +1,496 @@
+/*
+ * FP64 CPU oracle.  TEST INFRASTRUCTURE ONLY: see oracle.h.  The library never
+ * links or calls this file; tests, smoke() and bench.py's cpu_baseline leg do.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Hadamard matrices.                                                        */
+/* ------------------------------------------------------------------------ */
+
+/* Quadratic character of a (mod q), q prime: +1 residue, -1 non-residue, 0. */
+static int qchar(int a, int q)
+{
+    a %= q;
+    if (a < 0) a += q;
+    if (a == 0) return 0;
+    for (int x = 1; x < q; ++x)
+        if ((x * x) % q == a) return 1;
+    return -1;
+}
+
+/* Base blocks of order 12 and 20 (Paley construction from the quadratic
+ * character mod 11 and mod 19).  The arrangement (circulant in j-i for 12,
+ * back-circulant in i+j for 20, unit first row and column) reproduces the
+ * tables in hadamard_decoder.cpp:8-45 entry for entry; the golden fixture
+ * tests/golden/hadamard_base_tables.json pins that. */
+static double base_entry(int base, int i, int j)
+{
+    if (i == 0 || j == 0) return 1.0;
+    if (base == 12) {
+        if (i == j) return -1.0;
+        return (double)(-qchar(j - i, 11));
+    }
+    int m = (i + j - 2) % 19;
+    if (m == 0) return -1.0;
+    return (double)(-qchar(m, 19));
+}
+
+static int is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+/* hadamard_decoder.cpp:110-189 (generate_hadamard) and :191-222 (_sort_walsh).
+ * Sylvester doubling of a base block placed top-left: entry (i,j) is the base
+ * entry times (-1)^popcount((i/base) & (j/base)).  WALSH orders rows by the
+ * number of sign changes along the row (stable sort; the reference's
+ * std::sort only differs on ties, which occur for non-power-of-two orders). */
+int or_hadamard(int n, int ordering, double* H)
+{
+    if (n <= 0 || n > 1024) return -1;
+    int base;
+    if (is_pow2(n)) base = 1;
+    else if (n % 12 == 0 && is_pow2(n / 12)) base = 12;
+    else if (n % 20 == 0 && is_pow2(n / 20)) base = 20;
+    else return -1;
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            double b = base == 1 ? 1.0 : base_entry(base, i % base, j % base);
+            int bits = __builtin_popcount((unsigned)((i / base) & (j / base)));
+            H[(size_t)i * n + j] = (bits & 1) ? -b : b;
+        }
+    if (ordering == WALSH) {
+        int* changes = (int*)calloc(n, sizeof(int));
+        int* order = (int*)malloc(n * sizeof(int));
+        double* tmp = (double*)malloc((size_t)n * n * sizeof(double));
+        for (int i = 0; i < n; ++i) {
+            for (int j = 0; j + 1 < n; ++j)
+                changes[i] += (H[(size_t)i * n + j] * H[(size_t)i * n + j + 1] < 0);
+            order[i] = i;
+        }
+        for (int i = 1; i < n; ++i) { /* insertion sort: stable */
+            int v = order[i], j = i - 1;
+            while (j >= 0 && changes[order[j]] > changes[v]) { order[j + 1] = order[j]; --j; }
+            order[j + 1] = v;
+        }
+        for (int i = 0; i < n; ++i)
+            memcpy(tmp + (size_t)i * n, H + (size_t)order[i] * n, n * sizeof(double));
+        memcpy(H, tmp, (size_t)n * n * sizeof(double));
+        free(changes); free(order); free(tmp);
+    }
+    return 0;
+}
+
+/* hadamard_decoder.cpp:49-78.  The column-major SGEMM there computes
+ * out[j][i] = (1/T) * sum_k H[j][k] * conv[k][i] with i = c*S + s. */
+void or_decode(const double* conv, const double* H, int T, size_t cols, double* out)
+{
+    const double inv = 1.0 / (double)T;
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)cols; ++i) {
+        for (int j = 0; j < T; ++j) {
+            double acc = 0.0;
+            for (int k = 0; k < T; ++k) acc += H[(size_t)j * T + k] * conv[(size_t)k * cols + i];
+            out[(size_t)j * cols + i] = acc * inv;
+        }
+    }
+}
+
+/* data_conversion_kernels.cuh:25-41: out[t][c][s] = raw[map[c]][t*S + s]. */
+void or_convert(const void* raw, int type, const int16_t* map, size_t R0, int T, int C, int S, double* out)
+{
+#pragma omp parallel for schedule(static)
+    for (int t = 0; t < T; ++t)
+        for (int c = 0; c < C; ++c) {
+            size_t src = (size_t)(uint16_t)map[c] * R0 + (size_t)t * S;
+            double* dst = out + ((size_t)t * C + c) * S;
+            if (type == TYPE_I16) {
+                const int16_t* r = (const int16_t*)raw + src;
+                for (int s = 0; s < S; ++s) dst[s] = r[s];
+            } else {
+                const float* r = (const float*)raw + src;
+                for (int s = 0; s < S; ++s) dst[s] = r[s];
+            }
+        }
+}
+
+/* ------------------------------------------------------------------------ */
+/* DFT / FFT / Hilbert.                                                      */
+/* ------------------------------------------------------------------------ */
+
+/* exp(sign * 2*pi*i * m / n) with the angle reduced exactly in integers. */
+static void cis(long long m, long long n, int sign, double* c, double* s)
+{
+    m %= n;
+    if (m < 0) m += n;
+    double a = 2.0 * M_PI * (double)m / (double)n;
+    *c = cos(a);
+    *s = sign * sin(a);
+}
+
+/* Textbook O(n^2) DFT: X[k] = sum_n x[n] exp(sign*2*pi*i*k*n/N). */
+void or_dft(const double* re, const double* im, int n, int sign, double* ore, double* oim)
+{
+    for (int k = 0; k < n; ++k) {
+        double ar = 0.0, ai = 0.0;
+        for (int j = 0; j < n; ++j) {
+            double c, s;
+            cis((long long)k * j, n, sign, &c, &s);
+            double xr = re[j], xi = im ? im[j] : 0.0;
+            ar += xr * c - xi * s;
+            ai += xr * s + xi * c;
+        }
+        ore[k] = ar;
+        oim[k] = ai;
+    }
+}
+
+static void fft_pow2(double* re, double* im, int n, int sign)
+{
+    for (int i = 1, j = 0; i < n; ++i) {
+        int bit = n >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) {
+            double t = re[i]; re[i] = re[j]; re[j] = t;
+            t = im[i]; im[i] = im[j]; im[j] = t;
+        }
+    }
+    for (int len = 2; len <= n; len <<= 1) {
+        int half = len >> 1;
+        for (int k = 0; k < half; ++k) {
+            double wr, wi;
+            cis(k, len, sign, &wr, &wi);
+            for (int b = 0; b < n; b += len) {
+                int p = b + k, q = p + half;
+                double xr = re[q] * wr - im[q] * wi;
+                double xi = re[q] * wi + im[q] * wr;
+                re[q] = re[p] - xr; im[q] = im[p] - xi;
+                re[p] += xr; im[p] += xi;
+            }
+        }
+    }
+}
+
+/* In-place unnormalised DFT of any length: radix-2 for powers of two,
+ * Bluestein's chirp-z (via a power-of-two convolution) otherwise. */
+void or_fft(double* re, double* im, int n, int sign)
+{
+    if (n <= 1) return;
+    if (is_pow2(n)) { fft_pow2(re, im, n, sign); return; }
+    int m = 1;
+    while (m < 2 * n - 1) m <<= 1;
+    double* ar = (double*)calloc(m, sizeof(double));
+    double* ai = (double*)calloc(m, sizeof(double));
+    double* br = (double*)calloc(m, sizeof(double));
+    double* bi = (double*)calloc(m, sizeof(double));
+    double* wr = (double*)malloc(n * sizeof(double));
+    double* wi = (double*)malloc(n * sizeof(double));
+    for (int k = 0; k < n; ++k) { /* w_k = exp(sign*i*pi*k^2/n) */
+        cis((long long)k * k, 2LL * n, sign, &wr[k], &wi[k]);
+    }
+    for (int k = 0; k < n; ++k) {
+        ar[k] = re[k] * wr[k] - im[k] * wi[k];
+        ai[k] = re[k] * wi[k] + im[k] * wr[k];
+        br[k] = wr[k]; bi[k] = -wi[k];
+        if (k) { br[m - k] = wr[k]; bi[m - k] = -wi[k]; }
+    }
+    fft_pow2(ar, ai, m, -1);
+    fft_pow2(br, bi, m, -1);
+    for (int k = 0; k < m; ++k) {
+        double r = ar[k] * br[k] - ai[k] * bi[k];
+        double i = ar[k] * bi[k] + ai[k] * br[k];
+        ar[k] = r; ai[k] = i;
+    }
+    fft_pow2(ar, ai, m, 1);
+    for (int k = 0; k < n; ++k) {
+        double r = ar[k] / m, i = ai[k] / m;
+        re[k] = r * wr[k] - i * wi[k];
+        im[k] = r * wi[k] + i * wr[k];
+    }
+    free(ar); free(ai); free(br); free(bi); free(wr); free(wi);
+}
+
+/* hilbert_handler.cu:55-78: F = forward DFT_S of the taps zero-padded to S. */
+void or_filter_spectrum(const double* h, int L, int S, double* spec)
+{
+    double* re = (double*)calloc(S, sizeof(double));
+    double* im = (double*)calloc(S, sizeof(double));
+    for (int i = 0; i < L && i < S; ++i) re[i] = h[i];
+    or_fft(re, im, S, -1);
+    for (int k = 0; k < S; ++k) { spec[2 * k] = re[k]; spec[2 * k + 1] = im[k]; }
+    free(re); free(im);
+}
+
+/* hilbert_handler.cu:132-155 + hilbert_kernels.cuh:17-40.  Per row:
+ * X = DFT_S(x) (R2C, bins 0..S/2), Y[k] = w_k X[k] F[k] for k <= S/2 with
+ * w_0 = 0.75/S and w_k = 1.5/S otherwise (the Nyquist bin is not halved: the
+ * kernel halves bin S/2+1, which is already zero), Y[k] = 0 above S/2, and
+ * y = unnormalised inverse DFT_S(Y).  out is interleaved complex [rows][S]. */
+void or_hilbert(const double* x, size_t rows, int S, const double* h, int L, double* out)
+{
+    double* F = NULL;
+    if (L > 0) {
+        F = (double*)malloc(2 * (size_t)S * sizeof(double));
+        or_filter_spectrum(h, L, S, F);
+    }
+    const int half = S / 2;
+#pragma omp parallel
+    {
+        double* re = (double*)malloc(S * sizeof(double));
+        double* im = (double*)malloc(S * sizeof(double));
+#pragma omp for schedule(static)
+        for (long long r = 0; r < (long long)rows; ++r) {
+            const double* xr = x + (size_t)r * S;
+            for (int i = 0; i < S; ++i) { re[i] = xr[i]; im[i] = 0.0; }
+            or_fft(re, im, S, -1);
+            for (int k = 0; k < S; ++k) {
+                if (k > half) { re[k] = 0.0; im[k] = 0.0; continue; }
+                double w = (k == 0 ? 0.75 : 1.5) / (double)S;
+                double a = re[k] * w, b = im[k] * w;
+                if (F) {
+                    double fr = F[2 * k], fi = F[2 * k + 1];
+                    re[k] = a * fr - b * fi;
+                    im[k] = a * fi + b * fr;
+                } else {
+                    re[k] = a; im[k] = b;
+                }
+            }
+            or_fft(re, im, S, 1);
+            double* o = out + 2 * (size_t)r * S;
+            for (int i = 0; i < S; ++i) { o[2 * i] = re[i]; o[2 * i + 1] = im[i]; }
+        }
+        free(re); free(im);
+    }
+    free(F);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Delay and sum.                                                            */
+/* ------------------------------------------------------------------------ */
+
+/* beamformer.cu:17-98 (_params_to_constants), evaluated in float exactly as
+ * the reference host code does. */
+int or_das_constants(const CudaBeamformerParameters* bp, or_das_consts* k)
+{
+    memset(k, 0, sizeof(*k));
+    k->S = (int)bp->dec_data_dim[0];
+    k->C = (int)bp->dec_data_dim[1];
+    k->T = (int)bp->dec_data_dim[2];
+    k->spm = bp->sampling_frequency / bp->speed_of_sound;
+    float px = bp->xdc_element_pitch[0], py = bp->xdc_element_pitch[1];
+    k->pitch_x = px;
+    k->pitch_y = py;
+    k->xdc_min_x = -bp->xdc_transform[12] + px * 0.5f;
+    k->xdc_min_y = -bp->xdc_transform[13] + py * 0.5f;
+    k->xdc_max_x = bp->xdc_transform[12] - px * 0.5f;
+    k->xdc_max_y = bp->xdc_transform[13] - py * 0.5f;
+    k->delay = (int)(bp->time_offset * bp->sampling_frequency);
+    k->sequence = bp->das_shader_id;
+    k->Nx = (int)bp->output_points[0];
+    k->Ny = (int)bp->output_points[1];
+    k->Nz = (int)bp->output_points[2];
+    k->min_x = bp->output_min_coordinate[0];
+    k->min_y = bp->output_min_coordinate[1];
+    k->min_z = bp->output_min_coordinate[2];
+    k->res_x = (bp->output_max_coordinate[0] - bp->output_min_coordinate[0]) / (float)(bp->output_points[0] - 1);
+    k->res_y = k->res_x;
+    k->res_z = (bp->output_max_coordinate[2] - bp->output_min_coordinate[2]) / (float)(bp->output_points[2] - 1);
+    k->fn_rx = bp->fn_rx;
+    k->fn_tx = bp->fn_tx;
+    k->readi_groups = (uint8_t)bp->readi_group_count;
+    if (k->readi_groups == 0) k->readi_groups = 1;
+    k->readi_id = (uint8_t)bp->readi_group_id;
+    k->encoding = bp->decode;
+    float fz = bp->foci[2];
+    if (bp->tx_orientation == ORIENT_NONE) {
+        k->focus_type = 0;
+    } else if (isinf(fz) || bp->das_shader_id == RCA_TPW) {
+        fz = 0.0f;
+        k->focus_type = bp->tx_orientation == bp->rx_orientation ? 1 : 2;
+    } else {
+        k->focus_type = bp->tx_orientation == bp->rx_orientation ? 3 : 4;
+    }
+    k->focal_z = fz;
+    float cw = bp->coherency_weighting;
+    k->cw = cw < 0.0f ? 0.0f : (cw > 1.0f ? 1.0f : cw);
+    k->apo_type = bp->apo_type;
+    k->to_power = bp->to_power;
+    return 0;
+}
+
+/* READI sign row (beamformer.cu:238-252 / :355-372): bit i set when entry i of
+ * row readi_group_id of the group-sized decode matrix is negative. */
+int or_readi_row_bits(const or_das_consts* k, uint64_t* bits)
+{
+    *bits = 0;
+    if (k->readi_groups <= 1) return 0;
+    int g = k->readi_groups;
+    double* H = (double*)malloc((size_t)g * g * sizeof(double));
+    if (or_hadamard(g, k->encoding, H) != 0) { free(H); return -1; }
+    for (int i = 0; i < g && i < 64; ++i)
+        if (H[(size_t)k->readi_id * g + i] < 0) *bits |= (1ULL << i);
+    free(H);
+    return 0;
+}
+
+/* beamformer_utils.cuh:142-154 */
+static double hann_apo(double lateral, double depth, double fnum)
+{
+    double a = fnum * (lateral / depth) / 2.0;
+    if (a > 0.5) a = 0.5;
+    a = cos(M_PI * a);
+    return a * a;
+}
+
+/* templated_beamformer.cuh:86-99 */
+static double sin_to_apo(double pos, double peak, int power)
+{
+    double a = M_PI * pos / peak;
+    if (a < -M_PI) a = -M_PI;
+    if (a > M_PI) a = M_PI;
+    a = sin(a);
+    double out = 1.0;
+    for (int i = 0; i < power; ++i) out *= a;
+    return out;
+}
+
+/* beamformer_utils.cuh:99-140: Catmull-Rom (tangents (p[i+1]-p[i-1])/2) or
+ * linear read of an interleaved complex row.  The cubic form reads one sample
+ * past the row at x = S-2 with zero weight; that sample is taken as 0. */
+static void interp_row(const double* row, int S, double x, int linear, double* vr, double* vi)
+{
+    int i = (int)floor(x);
+    double t = x - i;
+#define P(j, c) (((j) >= 0 && (j) < S) ? row[2 * (size_t)(j) + (c)] : 0.0)
+    if (linear) {
+        *vr = P(i, 0) + t * (P(i + 1, 0) - P(i, 0));
+        *vi = P(i, 1) + t * (P(i + 1, 1) - P(i, 1));
+        return;
+    }
+    double t2 = t * t;
+    double h00 = (2.0 * t - 3.0) * t2 + 1.0;
+    double h10 = (t - 2.0) * t2 + t;
+    double h01 = (-2.0 * t + 3.0) * t2;
+    double h11 = (t - 1.0) * t2;
+    for (int c = 0; c < 2; ++c) {
+        double m0 = 0.5 * (P(i + 1, c) - P(i - 1, c));
+        double m1 = 0.5 * (P(i + 2, c) - P(i, c));
+        double v = h00 * P(i, c) + h01 * P(i + 1, c) + h10 * m0 + h11 * m1;
+        if (c == 0) *vr = v; else *vi = v;
+    }
+#undef P
+}
+
+static double clampd(double v, double lo, double hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+/* HERCULES: templated_beamformer.cuh:102-216; FORCES: :219-324. */
+int or_das(const double* iq, const or_das_consts* k, int interp, int cf_mode, double* out, long long* active)
+{
+    const int S = k->S, C = k->C, T = k->T;
+    const int G = k->readi_groups;
+    uint64_t row_bits = 0;
+    if (G > 1 && or_readi_row_bits(k, &row_bits) != 0) return -1;
+    const int hercules = k->sequence == HERCULES;
+    if (!hercules && k->sequence != FORCES) return -1;
+    if (hercules && k->focus_type != 2 && k->focus_type != 4) return -1;
+    if (!hercules && G > 1 && k->encoding != HADAMARD) return -1;
+    const long long nvox = (long long)k->Nx * k->Ny * k->Nz;
+    long long act_total = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : act_total)
+    for (long long v = 0; v < nvox; ++v) {
+        int ix = (int)(v % k->Nx);
+        int iy = (int)((v / k->Nx) % k->Ny);
+        int iz = (int)(v / ((long long)k->Nx * k->Ny));
+        double vx = (double)k->min_x + (double)ix * (double)k->res_x;
+        double vy = (double)k->min_y + (double)iy * (double)k->res_y;
+        double vz = (double)k->min_z + (double)iz * (double)k->res_z;
+        double ar = 0.0, ai = 0.0, inc = 0.0;
+        long long act = 0;
+        if (hercules) {
+            double fz = k->focal_z;
+            double tx_y = 0.0, tx_z = vz; /* YZ_PLANE with zero steering */
+            if (k->focus_type == 4) { tx_y = vy; tx_z = vz - fz; }
+            double txd = copysign(sqrt(tx_y * tx_y + tx_z * tx_z), tx_z) + fz;
+            for (int g = 0; g < (G > 1 ? G : 1); ++g)
+                for (int t = 0; t < T; ++t) {
+                    double cur = t;
+                    if (G > 1) cur = k->encoding == WALSH ? (double)t * G + g : (double)g * T + t;
+                    double ey = (double)k->xdc_min_y + cur * k->pitch_y;
+                    double ry = ey - vy;
+                    int neg = 0;
+                    if (G > 1) {
+                        int bit = k->encoding == WALSH ? ((t & 1) ? G - 1 - g : g) : g;
+                        neg = (int)((row_bits >> bit) & 1ULL);
+                    }
+                    for (int c = 0; c < C; ++c) {
+                        double ex = (double)k->xdc_min_x + (double)c * k->pitch_x;
+                        double rx = ex - vx;
+                        double apo;
+                        if (k->apo_type == RX_TO_SIN)
+                            apo = sin_to_apo(ex, k->xdc_max_x, k->to_power) * sin_to_apo(ey, k->xdc_max_y, k->to_power);
+                        else
+                            apo = hann_apo(sqrt(rx * rx + ry * ry), vz, k->fn_rx);
+                        if (!(apo > 0.1)) continue;
+                        ++act;
+                        double idx = (txd + sqrt(rx * rx + ry * ry + vz * vz) + fz) * (double)k->spm + k->delay;
+                        idx = clampd(idx, 1.0, S - 2.0);
+                        double pr, pi;
+                        interp_row(iq + 2 * ((size_t)t * C + c) * S, S, idx, interp, &pr, &pi);
+                        double a = neg ? -apo : apo;
+                        ar += a * pr; ai += a * pi;
+                        inc += pr * pr + pi * pi;
+                    }
+                }
+        } else {
+            for (int c = 0; c < C; ++c) {
+                double ec = (double)k->xdc_min_x + (double)c * k->pitch_x;
+                double rx = ec - vx;
+                double rxd = sqrt(rx * rx + vz * vz);
+                double rx_apo;
+                if (k->apo_type == RX_HANN || k->apo_type == TX_TO_SIN)
+                    rx_apo = hann_apo(fabs(rx), vz, k->fn_rx);
+                else
+                    rx_apo = sin_to_apo(ec, k->xdc_max_x, k->to_power);
+                if (!(rx_apo > 0.0)) continue;
+                for (int g = 0; g < G; ++g) {
+                    int neg = G > 1 ? (int)((row_bits >> g) & 1ULL) : 0;
+                    for (int t = 0; t < T; ++t) {
+                        double et = (double)k->xdc_min_x + (double)(g * T + t) * k->pitch_x;
+                        double tx = vx - et;
+                        double tx_apo;
+                        if (k->apo_type == BOTH_TO_SIN || k->apo_type == TX_TO_SIN)
+                            tx_apo = sin_to_apo(et, k->xdc_max_x, k->to_power);
+                        else
+                            tx_apo = hann_apo(fabs(tx), vz, k->fn_tx);
+                        double apo = rx_apo * tx_apo;
+                        ++act;
+                        double idx = (sqrt(tx * tx + vz * vz) + rxd) * (double)k->spm + k->delay;
+                        idx = clampd(idx, 1.0, S - 2.0);
+                        double pr, pi;
+                        interp_row(iq + 2 * ((size_t)t * C + c) * S, S, idx, interp, &pr, &pi);
+                        double a = neg ? -apo : apo;
+                        ar += a * pr; ai += a * pi;
+                        inc += pr * pr + pi * pi;
+                    }
+                }
+            }
+        }
+        if (!hercules || cf_mode == 1) {
+            double cf = pow((ar * ar + ai * ai) / inc, (double)k->cw);
+            if (isnan(cf)) cf = 0.0;
+            ar *= cf; ai *= cf;
+        }
+        out[2 * v] = ar;
+        out[2 * v + 1] = ai;
+        act_total += act;
+    }
+    if (active) *active = act_total;
+    return 0;
+}

@@ -1,0 +1,81 @@
+/*
+ * FP64 CPU oracle of the RF -> volume path and of the block-matching path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the checker that the HIP path is
+ * compared against; it is never linked into, called by or shipped with the
+ * library.  Only tests/, __graft_entry__.smoke() and the cpu_baseline leg of
+ * bench.py load it (through oracle/__init__.py).
+ *
+ * Every function restates the algorithm of a reference file (cited per
+ * function) in double precision.  Parameters that the reference derives in
+ * single precision on the host (samples per metre, voxel pitch, element
+ * positions) are derived the same way, in float, so the oracle evaluates the
+ * reference's geometry exactly and only the per-tap arithmetic is promoted.
+ */
+#ifndef CUT_ORACLE_H
+#define CUT_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include <stdbool.h>
+#include "../include/cuda_beamformer_parameters.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- Hadamard (cuda_toolkit/src/rf_processing/hadamard/hadamard_decoder.cpp) ---- */
+int or_hadamard(int n, int ordering, double* H);
+void or_decode(const double* conv, const double* H, int T, size_t cols, double* out);
+
+/* ---- convert (data_conversion_kernels.cuh:25-41) ---- */
+void or_convert(const void* raw, int type, const int16_t* map, size_t R0, int T, int C, int S, double* out);
+
+/* ---- FFT / Hilbert (hilbert_handler.cu:55-155, hilbert_kernels.cuh:17-40) ---- */
+void or_dft(const double* re, const double* im, int n, int sign, double* ore, double* oim);
+void or_fft(double* re, double* im, int n, int sign);
+void or_filter_spectrum(const double* h, int L, int S, double* spec /* 2*S interleaved */);
+void or_hilbert(const double* x, size_t rows, int S, const double* h, int L, double* out);
+
+/* ---- DAS (beamformer.cu:17-98, templated_beamformer.cuh:16-324, beamformer_utils.cuh:99-213) ---- */
+typedef struct or_das_consts {
+    int S, C, T;
+    int Nx, Ny, Nz;
+    float min_x, min_y, min_z;
+    float res_x, res_y, res_z;
+    float xdc_min_x, xdc_min_y, xdc_max_x, xdc_max_y;
+    float pitch_x, pitch_y;
+    float spm;
+    int delay;
+    float focal_z;
+    int focus_type; /* 0 none, 1 XZ_PLANE, 2 YZ_PLANE, 3 XZ_FOCUS, 4 YZ_FOCUS */
+    float fn_tx, fn_rx, cw;
+    int readi_groups, readi_id, encoding;
+    int apo_type, to_power;
+    int sequence;
+} or_das_consts;
+
+int or_das_constants(const CudaBeamformerParameters* bp, or_das_consts* k);
+/* interp: 0 = Catmull-Rom cubic (reference), 1 = linear.  cf_mode: 0 = reference
+ * behaviour (FORCES applies the coherence factor, HERCULES does not), 1 = apply
+ * it for HERCULES as well.  Returns the number of taps with apo above the
+ * sequence's threshold through *active (may be NULL).  out: interleaved complex
+ * [Nz][Ny][Nx]. */
+int or_das(const double* iq, const or_das_consts* k, int interp, int cf_mode, double* out, long long* active);
+int or_readi_row_bits(const or_das_consts* k, uint64_t* bits);
+
+/* ---- 2-D block matching (image_processor.cpp:106-325, block_match_kernels.cu) ---- */
+void or_zncc_valid(const double* src, int sw, int sh, int sstride,
+                   const double* tpl, int tw, int th, int tstride, double* out);
+int or_motion_detection(const float* images, const NCCMotionParameters* p, float* maps);
+
+/* ---- 3-D full-search ZNCC block matching ---- */
+void or_ncc3d_full(const float* tpl_vol, const float* src_vol, int nx, int ny, int nz,
+                   int block, int stride, int margin, int* disp, float* peak);
+double or_zncc3d_at(const float* tpl_vol, const float* src_vol, int nx, int ny, int nz,
+                    int bx, int by, int bz, int block, int dx, int dy, int dz);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

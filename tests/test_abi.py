@@ -1,0 +1,83 @@
+"""C-ABI checks that need no GPU: struct layout, exported symbols, host-side
+validation (every error path beamform() takes before touching the device)."""
+import ctypes as C
+import subprocess
+
+import numpy as np
+import pytest
+
+from cuda_ultrasound_toolkit_amd import _lib as L, api, configs
+
+
+def test_struct_layout_matches_reference():
+    BP = L.CudaBeamformerParameters
+    assert C.sizeof(BP) == 8948 and C.alignment(BP) == 4
+    off = {n: getattr(BP, n).offset for n, _ in BP._fields_}
+    assert off["rf_raw_dim"] == 72 and off["decode"] == 104 and off["interpolate"] == 192
+    assert off["coherency_weighting"] == 196 and off["channel_mapping"] == 200
+    assert off["foci"] == 1224 and off["tx_orientation"] == 4296 and off["filter_length"] == 4832
+    assert off["rf_filter"] == 4836 and off["sample_type"] == 8944
+    assert C.sizeof(L.NccMotionParameters) == 72
+    assert L.NccMotionParameters.use_subpixel.offset == 44
+    assert L.NccMotionParameters.image_dims.offset == 60
+    assert C.sizeof(L.CorrImagesParameters) == 16
+
+
+def test_library_exports_every_declared_symbol(lib):
+    out = subprocess.run(["nm", "-D", "--defined-only", str(L.LIB_PATH)], capture_output=True, text=True).stdout
+    names = {line.split()[-1] for line in out.splitlines() if line.strip()}
+    for sym in L.exported_symbols():
+        assert sym in names, sym
+        getattr(lib, sym)
+
+
+def test_header_declarations_are_exported():
+    """Every function declared in include/*.h is in the library."""
+    import re
+    from pathlib import Path
+    out = subprocess.run(["nm", "-D", "--defined-only", str(L.LIB_PATH)], capture_output=True, text=True).stdout
+    names = {line.split()[-1] for line in out.splitlines() if line.strip()}
+    inc = Path(__file__).resolve().parent.parent / "include"
+    pending = {"motion_detect_f32_matlab", "corr_images_f32_matlab", "motion_detection", "corr_images"}
+    for h in inc.glob("*.h"):
+        for m in re.finditer(r"EXPORT\w*\s+[\w\s\*]+?\b(\w+)\s*\(", h.read_text()):
+            if m.group(1) in pending:
+                continue
+            assert m.group(1) in names, (h.name, m.group(1))
+
+
+def _expect_false(acq, mutate, out_elems=None):
+    bp = acq.bp
+    mutate(bp)
+    n = bp.output_points[0] * bp.output_points[1] * bp.output_points[2]
+    out = np.zeros(n if out_elems is None else out_elems, np.complex64)
+    return api.beamform(acq.raw, out, bp)
+
+
+@pytest.mark.parametrize("case", [
+    "sample_type", "data_type", "bad_order", "map_range", "filter_long", "sequence",
+    "herc_focus", "zero_dims", "raw_short", "out_short", "fft_len"])
+def test_beamform_rejects_invalid_parameters(lib, case):
+    acq = configs.rca_small(generate=False)
+    mut = {
+        "sample_type": lambda b: setattr(b, "sample_type", L.SAMPLE_BANDPASS),
+        "data_type": lambda b: setattr(b, "data_type", L.TYPE_U8),
+        "bad_order": lambda b: (b.dec_data_dim.__setitem__(2, 36), b.rf_raw_dim.__setitem__(0, 36 * 1024)),
+        "map_range": lambda b: b.channel_mapping.__setitem__(3, 999),
+        "filter_long": lambda b: setattr(b, "filter_length", 1025),
+        "sequence": lambda b: setattr(b, "das_shader_id", L.RCA_VLS),
+        "herc_focus": lambda b: setattr(b, "tx_orientation", L.ORIENT_NONE),
+        "zero_dims": lambda b: b.dec_data_dim.__setitem__(0, 0),
+        "raw_short": lambda b: b.rf_raw_dim.__setitem__(1, 64),
+        "out_short": lambda b: None,
+        "fft_len": lambda b: (b.dec_data_dim.__setitem__(0, 1000), b.rf_raw_dim.__setitem__(0, 32 * 1000)),
+    }[case]
+    out_elems = 10 if case == "out_short" else None
+    assert _expect_false(acq, mut, out_elems) is False
+
+
+def test_c_api_call_order_errors(lib):
+    # decode / hilbert before buffers are registered
+    assert api.cuda_decode(0, 0) is False
+    assert api.cuda_hilbert(0, 1) is False
+    assert api.register_cuda_buffers(list(range(1, 18)), 99) is False  # > 16 buffers

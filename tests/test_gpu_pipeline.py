@@ -1,0 +1,247 @@
+"""HIP path vs the FP64 oracle, through the C ABI (runs on an MI355X).
+
+Bars: decode bit-exact; convert exact; Hilbert within 1e-5 relative L2;
+DAS and the whole beamform within 1e-4 relative L2 (FP32 arithmetic against
+FP64, the north-star tolerance for beamformed IQ).
+"""
+import numpy as np
+import pytest
+
+from cuda_ultrasound_toolkit_amd import _lib as L, api, configs
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(np.ravel(a - b)) / max(np.linalg.norm(np.ravel(b)), 1e-300))
+
+
+def _t(x, dev):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(x)).to(dev)
+
+
+def _rf_setup(acq):
+    bp = acq.bp
+    S, Cc, T = bp.dec_data_dim[0], bp.dec_data_dim[1], bp.dec_data_dim[2]
+    assert api.rf_init((bp.rf_raw_dim[0], bp.rf_raw_dim[1]), (S, Cc, T), bp.decode)
+    assert api.rf_set_channel_mapping(np.array(bp.channel_mapping[:Cc], np.int16))
+    return S, Cc, T
+
+
+@pytest.mark.parametrize("T,S,encoding", [(32, 1024, L.HADAMARD), (128, 512, L.HADAMARD), (16, 256, L.WALSH),
+                                          (64, 200, L.HADAMARD), (24, 128, L.HADAMARD), (40, 96, L.WALSH)])
+def test_decode_bit_exact(torch_dev, orc, T, S, encoding):
+    import torch
+    print(f"decode T={T} S={S} enc={encoding}", flush=True)
+    acq = configs.linear_2d(T=T, S=S, encoding=encoding, seed=T + S)
+    S_, Cc, T_ = _rf_setup(acq)
+    raw = _t(acq.raw, torch_dev)
+    H = orc.hadamard(T, encoding)
+    conv = orc.convert(acq.raw, L.TYPE_I16, acq.channel_map, T * S, T, Cc, S)
+    want = orc.decode(conv, H).astype(np.float32)
+    pow2 = (T & (T - 1)) == 0
+    outs = {}
+    for mode in (1, 2):  # FP32 MFMA GEMM, int8 MFMA hi/lo
+        assert api.set_decode_mode(mode)
+        out = torch.full((T * Cc * S,), np.nan, dtype=torch.float32, device=torch_dev)
+        assert api.rf_convert_decode(raw, L.TYPE_I16, out, 1, True)
+        torch.cuda.synchronize()
+        outs[mode] = out.cpu().numpy().reshape(T, Cc, S)
+        if pow2:
+            assert np.array_equal(outs[mode], want), f"mode {mode}"
+        else:  # 1/T inexact: one rounding of sum * fl(1/T)
+            assert np.allclose(outs[mode], want, rtol=2 ** -23, atol=0)
+    assert np.array_equal(outs[1], outs[2])  # int8 path is bit-exact to the FP32 path
+    assert api.set_decode_mode(0)
+    # into the real parts of a complex buffer, imaginary parts untouched
+    cplx = torch.full((T * Cc * S * 2,), 7.0, dtype=torch.float32, device=torch_dev)
+    assert api.rf_convert_decode(raw, L.TYPE_I16, cplx, 2, True)
+    torch.cuda.synchronize()
+    c = cplx.cpu().numpy()
+    assert np.array_equal(c[0::2].reshape(T, Cc, S), outs[2]) and np.all(c[1::2] == 7.0)
+
+
+def test_decode_full_int16_range(torch_dev, orc):
+    """Extreme int16 values (the hi/lo split edges) decode exactly."""
+    import torch
+    T, S = 32, 64
+    acq = configs.linear_2d(T=T, S=S, generate=False)
+    rng = np.random.default_rng(9)
+    raw = rng.choice(np.array([-32768, -32767, -129, -128, -1, 0, 1, 127, 128, 255, 256, 32639, 32640, 32767],
+                              np.int16), size=(T, T * S)).astype(np.int16)
+    acq.raw = raw
+    _rf_setup(acq)
+    H = orc.hadamard(T, L.HADAMARD)
+    want = orc.decode(orc.convert(raw, L.TYPE_I16, acq.channel_map, T * S, T, T, S), H).astype(np.float32)
+    for mode in (1, 2):
+        assert api.set_decode_mode(mode)
+        out = torch.zeros(T * T * S, dtype=torch.float32, device=torch_dev)
+        assert api.rf_convert_decode(_t(raw, torch_dev), L.TYPE_I16, out, 1, True)
+        torch.cuda.synchronize()
+        assert np.array_equal(out.cpu().numpy().reshape(T, T, S), want)
+    assert api.set_decode_mode(0)
+
+
+def test_convert_f32_and_i16(torch_dev, orc):
+    import torch
+    T, S = 8, 300
+    acq = configs.linear_2d(T=T, S=S, encoding=L.NONE, seed=5)
+    acq.bp.decode = L.NONE
+    _rf_setup(acq)
+    for dtype, arr in ((L.TYPE_I16, acq.raw), (L.TYPE_F32, acq.raw.astype(np.float32) * 0.37)):
+        out = torch.zeros(T * T * S, dtype=torch.float32, device=torch_dev)
+        assert api.rf_convert_decode(_t(arr, torch_dev), dtype, out, 1, False)
+        torch.cuda.synchronize()
+        want = orc.convert(arr, dtype, acq.channel_map, T * S, T, T, S)
+        assert np.array_equal(out.cpu().numpy().reshape(T, T, S), want.astype(np.float32))
+
+
+@pytest.mark.parametrize("S,taps", [(4096, 0), (4096, 64), (1024, 0), (1024, 17), (2048, 5), (256, 0),
+                                    (64, 3), (32, 0), (16, 1)])
+def test_hilbert_matches_oracle(torch_dev, orc, S, taps):
+    import torch
+    print(f"hilbert S={S} taps={taps}", flush=True)
+    T, Cc = 4, 3
+    acq = configs.linear_2d(T=T, S=S, generate=False)
+    acq.bp.dec_data_dim[1] = Cc
+    assert api.rf_init((T * S, Cc), (S, Cc, T), L.HADAMARD)
+    rng = np.random.default_rng(S + taps)
+    x = rng.standard_normal((T * Cc, S)).astype(np.float32) * 1000
+    h = rng.standard_normal(taps).astype(np.float32) if taps else None
+    assert api.rf_set_match_filter(h)
+    out = torch.zeros(T * Cc * S * 2, dtype=torch.float32, device=torch_dev)
+    assert api.rf_hilbert(_t(x, torch_dev), 1, out)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy().view(np.complex64).reshape(T * Cc, S)
+    want = orc.hilbert(x.astype(np.float64), None if h is None else h.astype(np.float64))
+    assert rel_l2(got, want) < 1e-5
+    # strided input: real parts of a complex buffer
+    xc = np.zeros((T * Cc, S), np.complex64)
+    xc.real = x
+    xc.imag = 123.0
+    out2 = torch.zeros_like(out)
+    assert api.rf_hilbert(_t(xc.view(np.float32), torch_dev), 2, out2)
+    torch.cuda.synchronize()
+    assert np.array_equal(out2.cpu().numpy(), out.cpu().numpy())
+    assert api.rf_set_match_filter(None)
+
+
+def _das_case(acq, orc, torch_dev, cubic, tol):
+    import torch
+    st = orc.pipeline(acq, cubic=cubic, return_stages=True)
+    iq = st["iq"].astype(np.complex64)
+    # reference volume from the same (float-rounded) IQ
+    want, active = orc.das(iq.astype(np.complex128), acq.bp, cubic=cubic)
+    assert api.set_das_options(cubic=cubic)
+    vol = torch.zeros(want.size * 2, dtype=torch.float32, device=torch_dev)
+    assert api.das_device(_t(iq.view(np.float32), torch_dev), vol, acq.bp)
+    torch.cuda.synchronize()
+    got = vol.cpu().numpy().view(np.complex64).reshape(want.shape)
+    assert api.set_das_options(cubic=True)
+    err = rel_l2(got, want)
+    print(f"  rel L2 {err:.3e}, active taps {active}")
+    assert err < tol
+    return got, want
+
+
+@pytest.mark.parametrize("cubic", [True, False])
+def test_das_hercules_matches_oracle(torch_dev, orc, cubic):
+    print(f"das hercules cubic={cubic}", flush=True)
+    acq = configs.rca_small(seed=21)
+    _das_case(acq, orc, torch_dev, cubic, 1e-4)
+
+
+def test_das_hercules_focused_and_ragged_grid(torch_dev, orc):
+    print("das hercules focus ragged", flush=True)
+    acq = configs.rca_small(seed=22, n=19, focus_z=8e-3)
+    acq.bp.output_points[:] = (19, 13, 21, 1)
+    _das_case(acq, orc, torch_dev, True, 1e-4)
+
+
+def test_das_hercules_sin_apodization(torch_dev, orc):
+    print("das hercules RX_TO_SIN", flush=True)
+    acq = configs.rca_small(seed=23)
+    acq.bp.apo_type = L.RX_TO_SIN
+    acq.bp.to_power = 2
+    _das_case(acq, orc, torch_dev, True, 1e-4)
+
+
+@pytest.mark.parametrize("cubic", [True, False])
+def test_das_forces_matches_oracle(torch_dev, orc, cubic):
+    print(f"das forces cubic={cubic}", flush=True)
+    acq = configs.linear_2d(seed=31)
+    _das_case(acq, orc, torch_dev, cubic, 1e-4)
+
+
+def test_das_forces_apodized_with_cf(torch_dev, orc):
+    print("das forces fn=1 cw=0.5", flush=True)
+    acq = configs.linear_2d(seed=32)
+    acq.bp.fn_tx = 1.0
+    acq.bp.fn_rx = 1.0
+    acq.bp.coherency_weighting = 0.5
+    _das_case(acq, orc, torch_dev, True, 1e-4)
+
+
+def test_beamform_api_linear_2d(lib, orc):
+    print("beamform linear_2d", flush=True)
+    acq = configs.linear_2d(seed=41)
+    got = api.beamform_volume(acq.raw, acq.bp)
+    assert got is not None
+    want = orc.pipeline(acq)
+    assert rel_l2(got, want) < 1e-4
+
+
+def test_beamform_api_rca_small(lib, orc):
+    print("beamform rca_small", flush=True)
+    acq = configs.rca_small(seed=42)
+    got = api.beamform_volume(acq.raw, acq.bp)
+    assert got is not None
+    want = orc.pipeline(acq)
+    assert rel_l2(got, want) < 1e-4
+    # decode = NONE takes the fused convert+Hilbert kernel
+    acq2 = configs.rca_small(seed=43, encoding=L.NONE)
+    acq2.bp.decode = L.NONE
+    got2 = api.beamform_volume(acq2.raw, acq2.bp)
+    assert rel_l2(got2, orc.pipeline(acq2)) < 1e-4
+
+
+def test_beamform_matlab_shim(lib, orc):
+    import ctypes as C
+    acq = configs.linear_2d(seed=44)
+    out = np.zeros(acq.volume_shape, np.complex64)
+    lib.beamform_i16_matlab(acq.raw.ctypes.data_as(C.POINTER(C.c_int16)), acq.bp,
+                            out.ctypes.data_as(C.POINTER(C.c_float)))
+    assert rel_l2(out, orc.pipeline(acq)) < 1e-4
+
+
+def test_gl_api_through_device_buffers(torch_dev, orc):
+    """cuda_decode / cuda_hilbert on registered device buffers (the GL seam)."""
+    import torch
+    print("gl api seam", flush=True)
+    T, S = 32, 1024
+    acq = configs.linear_2d(T=T, S=S, seed=51)
+    Cc = T
+    # raw buffer with a 64-byte header in front of the frame
+    raw = np.zeros(32 + acq.raw.size, np.int16)
+    raw[32:] = acq.raw.ravel()
+    d_raw = _t(raw, torch_dev)
+    bufs = [torch.zeros(T * Cc * S * 2, dtype=torch.float32, device=torch_dev) for _ in range(2)]
+    assert api.init_cuda_configuration((T * S, Cc), (S, Cc, T))
+    assert api.register_device_buffers(bufs, d_raw)
+    assert api.cuda_set_channel_mapping(acq.channel_map)
+    h = configs.chirp_filter(16)
+    assert api.cuda_set_match_filter(h)
+    assert api.cuda_decode(64, 0)
+    assert api.cuda_hilbert(0, 1)
+    torch.cuda.synchronize()
+    conv = orc.convert(acq.raw, L.TYPE_I16, acq.channel_map, T * S, T, Cc, S)
+    dec = orc.decode(conv, orc.hadamard(T, L.HADAMARD))
+    got_dec = bufs[0].cpu().numpy()[0::2].reshape(T, Cc, S)
+    assert np.array_equal(got_dec, dec.astype(np.float32))
+    got = bufs[1].cpu().numpy().view(np.complex64).reshape(T * Cc, S)
+    want = orc.hilbert(dec.reshape(T * Cc, S), h.astype(np.float64))
+    assert rel_l2(got, want) < 1e-5
+    assert api.cuda_hilbert(0, 5) is False  # unregistered index
+    api.deinit_cuda_configuration()
+    assert api.cuda_decode(0, 0) is False
