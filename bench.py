@@ -95,7 +95,10 @@ def main():
     nx, ny, nz = bp.output_points[0], bp.output_points[1], bp.output_points[2]
     d_raw = torch.from_numpy(acq.raw).to(dev)
     d_vol = torch.zeros(nx * ny * nz * 2, dtype=torch.float32, device=dev)
-    stream = torch.cuda.current_stream()
+    # a dedicated (non-null) stream: the library enqueues on it and the timing
+    # events are recorded on it
+    stream = torch.cuda.Stream(device=dev)
+    torch.cuda.set_stream(stream)
     api.set_stream(stream.cuda_stream)
     assert api.set_das_options(cubic=(args.interp == "cubic"))
 

@@ -19,6 +19,7 @@
 #include "common.hpp"
 
 #include <cmath>
+#include <type_traits>
 
 namespace cut {
 
@@ -29,6 +30,8 @@ constexpr int HW = 256;                         // window length (complex sample
 constexpr float HANN_A_THR = 0.39758361765043326f;
 
 __device__ __forceinline__ float cos_2pi(float x) { return __builtin_amdgcn_cosf(x); }
+// v_sqrt_f32 (1 ulp): the IEEE-exact sqrtf expansion costs ~15 VALU ops per call.
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 // cos^2(pi * min(a, 0.5)) = 0.5 + 0.5 cos(2 pi min(a, 0.5))  (beamformer_utils.cuh:142-154)
 __device__ __forceinline__ float hann_from_a(float a)
@@ -85,6 +88,30 @@ __device__ __forceinline__ float2 interp_global(const float2* __restrict__ row, 
         return cubic2(row_at(row, i - 1, S), row_at(row, i, S), row_at(row, i + 1, S), row_at(row, i + 2, S), t);
     else
         return lerp2(row_at(row, i, S), row_at(row, i + 1, S), t);
+}
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
+
+// Weight of one tap: apodization when it passes the reference's threshold
+// (apo > 0.1), else 0.  (Lanes of voxels outside the grid compute in-range
+// values that are never stored.)
+template <bool SINAPO>
+__device__ __forceinline__ float tap_weight(float rho, float kzj, float apo_c)
+{
+    float apo;
+    if constexpr (SINAPO) {
+        apo = apo_c;
+    } else {
+        const float a = __builtin_amdgcn_fmed3f(rho * kzj, -3.0e38f, 0.5f);  // min(a, 0.5)
+        apo = fmaf(0.5f, cos_2pi(a), 0.5f);
+    }
+    return apo > 0.1f ? apo : 0.0f;
+}
+
+// Fractional sample index (txd + |rx| + fz) * spm + delay, clamped to [1, S-2].
+__device__ __forceinline__ float tap_index(float r2, float spm, float basej, float hi_clamp)
+{
+    return __builtin_amdgcn_fmed3f(fmaf(fsqrt(r2), spm, basej), 1.0f, hi_clamp);
 }
 
 struct MinMax {
@@ -188,6 +215,11 @@ __global__ __launch_bounds__(256) void hercules_kernel(const float2* __restrict_
             }
             const float ey = fmaf(cur, p.pitch_y, p.xdc_min_y);
             const MinMax dyr = dist_range(ey, Y0, Y1);
+            // a negative READI sign is applied by accumulating this transmit into -acc
+            auto flip = [&]() {
+#pragma unroll
+                for (int j = 0; j < HT_Z; ++j) { acc[j].x = -acc[j].x; acc[j].y = -acc[j].y; }
+            };
             int c_lo = 0, c_hi = p.C - 1;
             if (aperture_bounded) {
                 if (dyr.lo >= rmax_ap) continue;
@@ -199,6 +231,7 @@ __global__ __launch_bounds__(256) void hercules_kernel(const float2* __restrict_
             const float dy = ey - vy;
             const float dy2 = dy * dy;
             const float2* trow = iq + (size_t)t * p.C * rowlen;
+            if (neg) flip();
             float apo_y = 1.0f;
             if constexpr (SINAPO) apo_y = sin_apo_to(ey, p.xdc_max_y, p.to_power);
 
@@ -235,50 +268,48 @@ __global__ __launch_bounds__(256) void hercules_kernel(const float2* __restrict_
                     const float ex = fmaf((float)c, p.pitch_x, p.xdc_min_x);
                     const float dx = ex - vx;
                     const float rho2 = fmaf(dx, dx, dy2);
-                    const int wlo = wlo_s[cc];
-                    const int mode = wmode_s[cc];
-                    const float2* wrow = win + cc * HW;
-                    const float2* grow = trow + (size_t)c * rowlen;
                     float rho = 0.0f, apo_c = 0.0f;
-                    if constexpr (SINAPO) apo_c = sin_apo_to(ex, p.xdc_max_x, p.to_power) * apo_y;
-                    else rho = sqrtf(rho2);
+                    if constexpr (SINAPO) {
+                        apo_c = sin_apo_to(ex, p.xdc_max_x, p.to_power) * apo_y;
+                        if (!(apo_c > 0.1f)) continue;  // uniform: depends on the element only
+                    } else {
+                        rho = fsqrt(rho2);
+                        // the deepest voxel of the column has the widest aperture:
+                        // skip the channel when it is outside for every lane
+                        if (!__any(rho * kz[HT_Z - 1] < HANN_A_THR * 1.001f || p.fn_rx <= 0.0f)) continue;
+                    }
+                    const int wlo = wlo_s[cc];
+                    if (wmode_s[cc] == 0) {
+                        const float2* wrow = win + cc * HW;
 #pragma unroll
-                    for (int j = 0; j < HT_Z; ++j) {
-                        float apo;
-                        if constexpr (SINAPO) {
-                            apo = apo_c;
-                        } else {
-                            const float a = rho * kz[j];
-                            if (!(a < HANN_A_THR * 1.001f) && p.fn_rx > 0.0f) continue;
-                            apo = hann_from_a(a);
-                        }
-                        if (apo > 0.1f && zvalid[j] && col_valid) {
-                            const float r = sqrtf(rho2 + z2[j]);
-                            float x = fmaf(r, p.spm, base[j]);
-                            x = fminf(fmaxf(x, lo_clamp), hi_clamp);
+                        for (int j = 0; j < HT_Z; ++j) {
+                            const float w = tap_weight<SINAPO>(rho, kz[j], apo_c);
+                            const float x = tap_index(rho2 + z2[j], p.spm, base[j], hi_clamp);
+                            const float tt = __builtin_amdgcn_fractf(x);
+                            const int li = clampi((int)x - wlo, CUBIC ? 1 : 0, CUBIC ? HW - 3 : HW - 2);
                             float2 v;
-                            if (mode == 0) {
-                                const float fi = floorf(x);
-                                const float tt = x - fi;
-                                int li = (int)fi - wlo;
-                                if constexpr (CUBIC) {
-                                    li = min(max(li, 1), HW - 3);
-                                    v = cubic2(wrow[li - 1], wrow[li], wrow[li + 1], wrow[li + 2], tt);
-                                } else {
-                                    li = min(max(li, 0), HW - 2);
-                                    v = lerp2(wrow[li], wrow[li + 1], tt);
-                                }
-                            } else {
-                                v = interp_global<CUBIC>(grow, x, p.S);
-                            }
-                            const float sa = neg ? -apo : apo;
-                            acc[j].x = fmaf(sa, v.x, acc[j].x);
-                            acc[j].y = fmaf(sa, v.y, acc[j].y);
+                            if constexpr (CUBIC) v = cubic2(wrow[li - 1], wrow[li], wrow[li + 1], wrow[li + 2], tt);
+                            else v = lerp2(wrow[li], wrow[li + 1], tt);
+                            acc[j].x = fmaf(w, v.x, acc[j].x);
+                            acc[j].y = fmaf(w, v.y, acc[j].y);
+                            if constexpr (CF) inc[j] = (w != 0.0f) ? fmaf(v.x, v.x, fmaf(v.y, v.y, inc[j])) : inc[j];
+                        }
+                    } else {
+                        const float2* grow = trow + (size_t)c * rowlen;
+#pragma unroll
+                        for (int j = 0; j < HT_Z; ++j) {
+                            const float w = tap_weight<SINAPO>(rho, kz[j], apo_c);
+                            if (w == 0.0f) continue;
+                            const float x = tap_index(rho2 + z2[j], p.spm, base[j], hi_clamp);
+                            const float2 v = interp_global<CUBIC>(grow, x, p.S);
+                            acc[j].x = fmaf(w, v.x, acc[j].x);
+                            acc[j].y = fmaf(w, v.y, acc[j].y);
                             if constexpr (CF) inc[j] = fmaf(v.x, v.x, fmaf(v.y, v.y, inc[j]));
                         }
                     }
                 }
             }
+            if (neg) flip();
         }
     }
     if (!col_valid) return;
@@ -317,7 +348,7 @@ __global__ __launch_bounds__(256) void forces_kernel(const float2* __restrict__ 
     for (int c = 0; c < p.C; ++c) {
         const float ec = fmaf((float)c, p.pitch_x, p.xdc_min_x);
         const float rx = ec - vx;
-        const float rxd = sqrtf(fmaf(rx, rx, vz2));
+        const float rxd = fsqrt(fmaf(rx, rx, vz2));
         float rx_apo;
         if (p.apo_type == RX_HANN || p.apo_type == TX_TO_SIN) rx_apo = hann_from_a(p.fn_rx * (fabsf(rx) / vz) * 0.5f);
         else rx_apo = sin_apo_to(ec, p.xdc_max_x, p.to_power);
@@ -331,7 +362,7 @@ __global__ __launch_bounds__(256) void forces_kernel(const float2* __restrict__ 
                 if (p.apo_type == BOTH_TO_SIN || p.apo_type == TX_TO_SIN) tx_apo = sin_apo_to(et, p.xdc_max_x, p.to_power);
                 else tx_apo = hann_from_a(p.fn_tx * (fabsf(tx) / vz) * 0.5f);
                 float apo = rx_apo * tx_apo;
-                float x = fmaf(sqrtf(fmaf(tx, tx, vz2)) + rxd, p.spm, (float)p.delay);
+                float x = fmaf(fsqrt(fmaf(tx, tx, vz2)) + rxd, p.spm, (float)p.delay);
                 x = fminf(fmaxf(x, 1.0f), (float)p.S - 2.0f);
                 const float2 val = interp_global<CUBIC>(iq + ((size_t)t * p.C + c) * p.S, x, p.S);
                 if (neg) apo = -apo;
