@@ -28,6 +28,7 @@ BeamformState& bf_state()
 
 struct StageState {
     cut::RfPipeline rf;
+    cut::DeviceBuffer das_ws;
 };
 
 StageState& stage_state()
@@ -201,7 +202,10 @@ bool cut_das_device(const void* d_iq, void* d_volume, const CudaBeamformerParame
         cut::DasOptions o;
         o.cubic = cut::options().das_cubic;
         o.cf_hercules = cut::options().das_cf_hercules;
-        return cut::launch_das(d_iq, 0, 1.0f, p, o, (float2*)d_volume, cut::stream());
+        auto& ws = stage_state().das_ws;
+        const size_t need = cut::das_workspace_bytes(p);
+        if (need && !ws.ensure(std::max(need, ws.bytes))) return false;
+        return cut::launch_das(d_iq, 0, 1.0f, p, o, (float2*)d_volume, ws.ptr, ws.bytes, cut::stream());
     });
 }
 

@@ -128,7 +128,11 @@ struct DasOptions {
     int cubic = 1;         // 1 = Catmull-Rom (reference), 0 = linear
     int cf_hercules = 0;   // apply the coherence factor in HERCULES as well
 };
+// HERCULES splits the transmit loop over workgroups and needs a workspace for
+// the partial sums (das_workspace_bytes; 0 when no split is used).
+int das_splits(long long tiles, int gt_total);
+size_t das_workspace_bytes(const DasParams& p);
 bool launch_das(const void* iq, int iq_kind, float iq_scale, const DasParams& p, const DasOptions& o,
-                float2* vol, hipStream_t st);
+                float2* vol, void* workspace, size_t workspace_bytes, hipStream_t st);
 
 }  // namespace cut

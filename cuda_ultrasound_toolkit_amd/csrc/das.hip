@@ -19,6 +19,7 @@
 #include "common.hpp"
 
 #include <cmath>
+#include <algorithm>
 #include <type_traits>
 
 namespace cut {
@@ -126,9 +127,19 @@ __device__ __forceinline__ MinMax dist_range(float e, float a, float b)
     return {dmin, dmax};
 }
 
+// Work decomposition: tiles (nxt x nyt x nzt) times `splits` contiguous ranges
+// of the (group, transmit) loop, one workgroup each, deepest tiles first (the
+// active-tap count grows with depth, so heavy workgroups start first and the
+// light ones fill the tail).  With splits > 1 every workgroup writes its partial
+// sums (acc.re, acc.im, incoherent) and hercules_reduce adds them in split order.
+struct HercGrid {
+    int nxt, nyt, nzt, splits, gt_per_split;
+};
+
 template <bool CUBIC, bool FOCUS, bool CF, bool SINAPO>
 __global__ __launch_bounds__(256) void hercules_kernel(const float2* __restrict__ iq, const DasParams p,
-                                                       float2* __restrict__ vol)
+                                                       const HercGrid hg, float2* __restrict__ vol,
+                                                       float4* __restrict__ part)
 {
     __shared__ float2 win[HCB * HW];
     __shared__ int wlo_s[HCB];
@@ -136,7 +147,12 @@ __global__ __launch_bounds__(256) void hercules_kernel(const float2* __restrict_
 
     const int tid = threadIdx.x;
     const int lx = tid % HT_X, ly = tid / HT_X;
-    const int bx = blockIdx.x * HT_X, by = blockIdx.y * HT_Y, bz = blockIdx.z * HT_Z;
+    const int per_z = hg.nxt * hg.nyt * hg.splits;
+    const int wz = blockIdx.x / per_z;
+    const int wr = blockIdx.x - wz * per_z;
+    const int split = wr % hg.splits;
+    const int wxy = wr / hg.splits;
+    const int bx = (wxy % hg.nxt) * HT_X, by = (wxy / hg.nxt) * HT_Y, bz = (hg.nzt - 1 - wz) * HT_Z;
     const int ix = bx + lx, iy = by + ly;
     const bool col_valid = ix < p.Nx && iy < p.Ny;
 
@@ -204,8 +220,11 @@ __global__ __launch_bounds__(256) void hercules_kernel(const float2* __restrict_
     const float lo_clamp = 1.0f, hi_clamp = (float)p.S - 2.0f;
     const size_t rowlen = (size_t)p.S;
 
-    for (int g = 0; g < G; ++g) {
-        for (int t = 0; t < p.T; ++t) {
+    const int gt_lo = split * hg.gt_per_split;
+    const int gt_hi = min(G * p.T, gt_lo + hg.gt_per_split);
+    for (int gt = gt_lo; gt < gt_hi; ++gt) {
+        {
+            const int g = gt / p.T, t = gt - g * p.T;
             float cur = (float)t;
             bool neg = false;
             if (G > 1) {
@@ -317,15 +336,43 @@ __global__ __launch_bounds__(256) void hercules_kernel(const float2* __restrict_
     for (int j = 0; j < HT_Z; ++j) {
         const int iz = bz + j;
         if (iz >= p.Nz) break;
+        const size_t vi = ((size_t)iz * p.Ny + iy) * p.Nx + ix;
         float2 a = acc[j];
+        if (hg.splits > 1) {
+            part[(size_t)split * p.Nx * p.Ny * p.Nz + vi] = make_float4(a.x, a.y, inc[j], 0.0f);
+            continue;
+        }
         if constexpr (CF) {
             float cf = powf((a.x * a.x + a.y * a.y) / inc[j], p.cw);
             if (isnan(cf)) cf = 0.0f;
             a.x *= cf;
             a.y *= cf;
         }
-        vol[((size_t)iz * p.Ny + iy) * p.Nx + ix] = a;
+        vol[vi] = a;
     }
+}
+
+// Sum of the per-split partials in split order (deterministic), then the
+// coherence factor when it applies.
+__global__ __launch_bounds__(256) void hercules_reduce(const float4* __restrict__ part, int splits, size_t nvox,
+                                                       int apply_cf, float cw, float2* __restrict__ vol)
+{
+    const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nvox) return;
+    float ax = 0.0f, ay = 0.0f, inc = 0.0f;
+    for (int s = 0; s < splits; ++s) {
+        const float4 q = part[(size_t)s * nvox + v];
+        ax += q.x;
+        ay += q.y;
+        inc += q.z;
+    }
+    if (apply_cf) {
+        float cf = powf((ax * ax + ay * ay) / inc, cw);
+        if (isnan(cf)) cf = 0.0f;
+        ax *= cf;
+        ay *= cf;
+    }
+    vol[v] = make_float2(ax, ay);
 }
 
 // FORCES: one thread per voxel (the volume is y-invariant in the reference's
@@ -377,8 +424,26 @@ __global__ __launch_bounds__(256) void forces_kernel(const float2* __restrict__ 
     vol[v] = make_float2(acc.x * cf, acc.y * cf);
 }
 
+size_t das_workspace_bytes(const DasParams& p)
+{
+    if (p.sequence != HERCULES) return 0;
+    const long long tiles = (long long)((p.Nx + HT_X - 1) / HT_X) * ((p.Ny + HT_Y - 1) / HT_Y) * ((p.Nz + HT_Z - 1) / HT_Z);
+    const int G = p.readi_groups > 1 ? p.readi_groups : 1;
+    const int splits = das_splits(tiles, G * p.T);
+    return splits > 1 ? (size_t)splits * p.Nx * p.Ny * p.Nz * sizeof(float4) : 0;
+}
+
+int das_splits(long long tiles, int gt_total)
+{
+    // aim for >= 4096 workgroups (4 resident per CU x 256 CUs x 4 rounds), at
+    // least 16 transmits per split
+    long long s = (4096 + tiles - 1) / tiles;
+    s = std::min<long long>(s, std::max(1, gt_total / 16));
+    return (int)std::max<long long>(1, std::min<long long>(s, 16));
+}
+
 bool launch_das(const void* iq, int iq_kind, float iq_scale, const DasParams& p, const DasOptions& o, float2* vol,
-                hipStream_t st)
+                void* workspace, size_t workspace_bytes, hipStream_t st)
 {
     (void)iq_scale;
     if (iq_kind != 0) CUT_FAIL("DAS: staged IQ formats are not enabled in this build");
@@ -386,11 +451,27 @@ bool launch_das(const void* iq, int iq_kind, float iq_scale, const DasParams& p,
     if (p.Nx <= 0 || p.Ny <= 0 || p.Nz <= 0) CUT_FAIL("DAS: empty output grid");
     if (p.sequence == HERCULES) {
         if (p.focus_type != 2 && p.focus_type != 4) CUT_FAIL("Invalid focus for HERCULES.");
-        dim3 grid((p.Nx + HT_X - 1) / HT_X, (p.Ny + HT_Y - 1) / HT_Y, (p.Nz + HT_Z - 1) / HT_Z);
+        HercGrid hg;
+        hg.nxt = (p.Nx + HT_X - 1) / HT_X;
+        hg.nyt = (p.Ny + HT_Y - 1) / HT_Y;
+        hg.nzt = (p.Nz + HT_Z - 1) / HT_Z;
+        const long long tiles = (long long)hg.nxt * hg.nyt * hg.nzt;
+        const int G = p.readi_groups > 1 ? p.readi_groups : 1;
+        hg.splits = das_splits(tiles, G * p.T);
+        hg.gt_per_split = (G * p.T + hg.splits - 1) / hg.splits;
+        const size_t nvox = (size_t)p.Nx * p.Ny * p.Nz;
+        float4* part = nullptr;
+        if (hg.splits > 1) {
+            if (!workspace || workspace_bytes < (size_t)hg.splits * nvox * sizeof(float4))
+                CUT_FAIL("DAS: workspace of " << (size_t)hg.splits * nvox * sizeof(float4) << " bytes needed");
+            part = (float4*)workspace;
+        }
+        if (tiles * hg.splits > 0x7fffffffLL) CUT_FAIL("DAS: grid too large");
+        dim3 grid((unsigned)(tiles * hg.splits));
         const bool focus = p.focus_type == 4;
         const bool sinapo = p.apo_type == RX_TO_SIN;
         const bool cf = o.cf_hercules != 0;
-#define CUT_HERC(CU, FO, CFV, SA) hercules_kernel<CU, FO, CFV, SA><<<grid, 256, 0, st>>>(d, p, vol)
+#define CUT_HERC(CU, FO, CFV, SA) hercules_kernel<CU, FO, CFV, SA><<<grid, 256, 0, st>>>(d, p, hg, vol, part)
 #define CUT_HERC_SA(CU, FO, CFV) do { if (sinapo) CUT_HERC(CU, FO, CFV, true); else CUT_HERC(CU, FO, CFV, false); } while (0)
 #define CUT_HERC_CF(CU, FO) do { if (cf) CUT_HERC_SA(CU, FO, true); else CUT_HERC_SA(CU, FO, false); } while (0)
 #define CUT_HERC_FO(CU) do { if (focus) CUT_HERC_CF(CU, true); else CUT_HERC_CF(CU, false); } while (0)
@@ -399,6 +480,10 @@ bool launch_das(const void* iq, int iq_kind, float iq_scale, const DasParams& p,
 #undef CUT_HERC_CF
 #undef CUT_HERC_SA
 #undef CUT_HERC
+        CUT_HIP_CHECK(hipGetLastError());
+        if (hg.splits > 1) {
+            hercules_reduce<<<(unsigned)((nvox + 255) / 256), 256, 0, st>>>(part, hg.splits, nvox, cf ? 1 : 0, p.cw, vol);
+        }
     } else if (p.sequence == FORCES) {
         if (p.readi_groups > 1 && p.encoding != HADAMARD) CUT_FAIL("FORCES only supports Hadamard encoding at this time.");
         const long long nvox = (long long)p.Nx * p.Ny * p.Nz;

@@ -466,7 +466,9 @@ bool beamform_device(RfPipeline& rf, BeamformBuffers& bufs, const void* d_raw, s
     DasOptions o;
     o.cubic = options().das_cubic;
     o.cf_hercules = options().das_cf_hercules;
-    if (!launch_das(iq, 0, 1.0f, p, o, d_vol, st)) CUT_FAIL("Beamforming failed.");
+    const size_t ws = das_workspace_bytes(p);
+    if (ws && !bufs.das_ws.ensure(std::max(ws, bufs.das_ws.bytes))) return false;
+    if (!launch_das(iq, 0, 1.0f, p, o, d_vol, bufs.das_ws.ptr, bufs.das_ws.bytes, st)) CUT_FAIL("Beamforming failed.");
     return true;
 }
 

@@ -62,7 +62,7 @@ private:
 
 // Whole path raw -> volume on device pointers (the body of beamform()).
 struct BeamformBuffers {
-    DeviceBuffer raw, dec, iq, vol;
+    DeviceBuffer raw, dec, iq, vol, das_ws;
 };
 // Every check beamform() makes before touching the GPU (host only).
 bool validate_beamform(const CudaBeamformerParameters& bp, size_t raw_bytes, size_t vol_bytes);

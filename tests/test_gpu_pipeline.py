@@ -127,18 +127,18 @@ def test_hilbert_matches_oracle(torch_dev, orc, S, taps):
     assert api.rf_set_match_filter(None)
 
 
-def _das_case(acq, orc, torch_dev, cubic, tol):
+def _das_case(acq, orc, torch_dev, cubic, tol, cf=False):
     import torch
     st = orc.pipeline(acq, cubic=cubic, return_stages=True)
     iq = st["iq"].astype(np.complex64)
     # reference volume from the same (float-rounded) IQ
-    want, active = orc.das(iq.astype(np.complex128), acq.bp, cubic=cubic)
-    assert api.set_das_options(cubic=cubic)
+    want, active = orc.das(iq.astype(np.complex128), acq.bp, cubic=cubic, cf_hercules=cf)
+    assert api.set_das_options(cubic=cubic, cf_hercules=cf)
     vol = torch.zeros(want.size * 2, dtype=torch.float32, device=torch_dev)
     assert api.das_device(_t(iq.view(np.float32), torch_dev), vol, acq.bp)
     torch.cuda.synchronize()
     got = vol.cpu().numpy().view(np.complex64).reshape(want.shape)
-    assert api.set_das_options(cubic=True)
+    assert api.set_das_options(cubic=True, cf_hercules=False)
     err = rel_l2(got, want)
     print(f"  rel L2 {err:.3e}, active taps {active}")
     assert err < tol
@@ -156,6 +156,20 @@ def test_das_hercules_focused_and_ragged_grid(torch_dev, orc):
     print("das hercules focus ragged", flush=True)
     acq = configs.rca_small(seed=22, n=19, focus_z=8e-3)
     acq.bp.output_points[:] = (19, 13, 21, 1)
+    _das_case(acq, orc, torch_dev, True, 1e-4)
+
+
+def test_das_hercules_coherence_factor_extension(torch_dev, orc):
+    print("das hercules cf", flush=True)
+    acq = configs.rca_small(seed=24)
+    acq.bp.coherency_weighting = 0.7
+    _das_case(acq, orc, torch_dev, True, 1e-4, cf=True)
+
+
+def test_das_hercules_single_tile_no_split(torch_dev, orc):
+    """A grid of one tile with few transmits takes the unsplit path."""
+    print("das hercules unsplit", flush=True)
+    acq = configs.rca_small(seed=25, T=16, n=8)
     _das_case(acq, orc, torch_dev, True, 1e-4)
 
 
