@@ -74,6 +74,8 @@ def main():
     ap.add_argument("--interp", choices=["linear", "cubic"], default="linear")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--stage-iters", type=int, default=10)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one device-to-device copy of the 512 MiB IQ buffer (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); frames shard with no exchange, "
@@ -145,6 +147,11 @@ def main():
     api.set_das_options(cubic=(other_interp == "cubic"))
     ms_das_other = timed(lambda: api.das_device(d_iq, d_vol, bp), 3)
     api.set_das_options(cubic=(args.interp == "cubic"))
+    if args.copy_calibration:
+        d_copy = torch.empty_like(d_iq)
+        d_copy.copy_(d_iq)
+        torch.cuda.synchronize()
+        log(f"bench: calibration copy of {d_iq.numel() * 4} bytes")
     api.set_stream(None)
 
     taps = C.c_ulonglong()

@@ -99,14 +99,14 @@ __device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v,
 template <bool SINAPO>
 __device__ __forceinline__ float tap_weight(float rho, float kzj, float apo_c)
 {
-    float apo;
     if constexpr (SINAPO) {
-        apo = apo_c;
+        return apo_c > 0.1f ? apo_c : 0.0f;
     } else {
-        const float a = __builtin_amdgcn_fmed3f(rho * kzj, -3.0e38f, 0.5f);  // min(a, 0.5)
-        apo = fmaf(0.5f, cos_2pi(a), 0.5f);
+        // cos^2(pi a) > 0.1 <=> a < acos(sqrt(0.1))/pi on [0, 0.5]; for those a
+        // the reference's min(a, 0.5) clamp is inactive
+        const float a = rho * kzj;
+        return fabsf(a) < HANN_A_THR ? fmaf(0.5f, cos_2pi(a), 0.5f) : 0.0f;
     }
-    return apo > 0.1f ? apo : 0.0f;
 }
 
 // Fractional sample index (txd + |rx| + fz) * spm + delay, clamped to [1, S-2].
@@ -300,12 +300,17 @@ __global__ __launch_bounds__(256) void hercules_kernel(const float2* __restrict_
                     const int wlo = wlo_s[cc];
                     if (wmode_s[cc] == 0) {
                         const float2* wrow = win + cc * HW;
+                        // indices relative to the window: base - wlo, clamp bounds shifted alike
+                        const float fwlo = (float)wlo;
+                        const float lo_w = lo_clamp - fwlo, hi_w = hi_clamp - fwlo;
 #pragma unroll
                         for (int j = 0; j < HT_Z; ++j) {
                             const float w = tap_weight<SINAPO>(rho, kz[j], apo_c);
-                            const float x = tap_index(rho2 + z2[j], p.spm, base[j], hi_clamp);
+                            // the window covers every voxel of the tile with a 2-sample margin,
+                            // so the local index needs no further clamp
+                            const float x = __builtin_amdgcn_fmed3f(fmaf(fsqrt(rho2 + z2[j]), p.spm, base[j] - fwlo), lo_w, hi_w);
                             const float tt = __builtin_amdgcn_fractf(x);
-                            const int li = clampi((int)x - wlo, CUBIC ? 1 : 0, CUBIC ? HW - 3 : HW - 2);
+                            const int li = (int)x;
                             float2 v;
                             if constexpr (CUBIC) v = cubic2(wrow[li - 1], wrow[li], wrow[li + 1], wrow[li + 2], tt);
                             else v = lerp2(wrow[li], wrow[li + 1], tt);

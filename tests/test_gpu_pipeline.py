@@ -259,3 +259,31 @@ def test_gl_api_through_device_buffers(torch_dev, orc):
     assert api.cuda_hilbert(0, 5) is False  # unregistered index
     api.deinit_cuda_configuration()
     assert api.cuda_decode(0, 0) is False
+
+
+@pytest.mark.parametrize("cubic", [False, True])
+def test_full_size_rca3d_depth_planes(torch_dev, orc, cubic):
+    """Full 128 tx x 128 ch x 4096 frame with the 64-tap filter, beamformed on
+    4 depth planes (iz = 16, 48, 80, 112 of the 128^3 grid) by the GPU and by
+    the FP64 oracle: relative L2 within 1e-4."""
+    import torch
+    print(f"full-size rca_3d planes cubic={cubic}", flush=True)
+    acq = configs.rca_3d(seed=2)
+    bp = acq.bp
+    nz = bp.output_points[2]
+    zmin, zmax = bp.output_min_coordinate[2], bp.output_max_coordinate[2]
+    res = np.float32((np.float32(zmax) - np.float32(zmin)) / np.float32(nz - 1))
+    bp.output_points[2] = 4
+    bp.output_min_coordinate[2] = float(np.float32(zmin) + np.float32(16) * res)
+    bp.output_max_coordinate[2] = float(np.float32(zmin) + np.float32(112) * res)
+    assert api.set_das_options(cubic=cubic)
+    d_raw = _t(acq.raw, torch_dev)
+    vol = torch.zeros(128 * 128 * 4 * 2, dtype=torch.float32, device=torch_dev)
+    assert api.beamform_device(d_raw, vol, bp)
+    torch.cuda.synchronize()
+    assert api.set_das_options(cubic=True)
+    got = vol.cpu().numpy().view(np.complex64).reshape(4, 128, 128)
+    want = orc.pipeline(acq, cubic=cubic)
+    err = rel_l2(got, want)
+    print(f"  rel L2 {err:.3e}; max |err| / max |v| {np.abs(got - want).max() / np.abs(want).max():.3e}")
+    assert err < 1e-4
