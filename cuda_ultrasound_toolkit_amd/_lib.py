@@ -139,6 +139,10 @@ def load() -> C.CDLL:
     B, V, P = C.c_bool, C.c_void_p, C.POINTER
     u = C.c_uint
     _proto(lib, SYM_BEAMFORM, B, [Span, Span, P(CudaBeamformerParameters)])
+    _proto(lib, SYM_MOTION, B, [Span, Span, P(NccMotionParameters)])
+    _proto(lib, SYM_CORR, B, [Span, Span, Span, P(u), P(u)])
+    _proto(lib, "motion_detect_f32_matlab", None, [P(C.c_float), NccMotionParameters, P(C.c_float)])
+    _proto(lib, "corr_images_f32_matlab", None, [P(C.c_float), P(C.c_float), CorrImagesParameters, P(C.c_float)])
     _proto(lib, "init_cuda_configuration", B, [P(u), P(u)])
     _proto(lib, "register_cuda_buffers", B, [P(u), u, u])
     _proto(lib, "cuda_set_channel_mapping", B, [P(C.c_int16)])
@@ -170,7 +174,7 @@ def load() -> C.CDLL:
 def exported_symbols() -> list[str]:
     """Names every header under include/ declares (checked by the CPU test suite)."""
     return [
-        SYM_BEAMFORM,
+        SYM_BEAMFORM, SYM_MOTION, SYM_CORR, "motion_detect_f32_matlab", "corr_images_f32_matlab",
         "init_cuda_configuration", "register_cuda_buffers", "cuda_set_channel_mapping",
         "cuda_set_match_filter", "cuda_hilbert", "cuda_decode", "deinit_cuda_configuration",
         "beamform_i16_matlab", "beamform_f32_matlab",

@@ -34,6 +34,34 @@ def beamform_volume(raw: np.ndarray, bp: L.CudaBeamformerParameters) -> np.ndarr
     return out if beamform(np.ascontiguousarray(raw), out, bp) else None
 
 
+def motion_detection(images: np.ndarray, motion_maps: np.ndarray, params: L.NccMotionParameters) -> bool:
+    """cuda_toolkit::motion_detection: f32 frames [F][H][W] -> float4 maps [F][gy][gx][4]."""
+    lib = L.load()
+    return bool(getattr(lib, L.SYM_MOTION)(_span(images), _span(motion_maps), C.byref(params)))
+
+
+def corr_images(template: np.ndarray, source: np.ndarray, corr_map: np.ndarray) -> bool:
+    """cuda_toolkit::corr_images on f32 images [H][W]; corr_map [Hs-Ht+1][Ws-Wt+1]."""
+    lib = L.load()
+    td = (C.c_uint * 2)(template.shape[1], template.shape[0])
+    sd = (C.c_uint * 2)(source.shape[1], source.shape[0])
+    return bool(getattr(lib, L.SYM_CORR)(_span(template), _span(source), _span(corr_map), td, sd))
+
+
+def motion_params(W, H, frames, grid, spacing, patch, margins, abs_thr=0.5, rel_thr=0.05, min_var=0.0,
+                  subpixel=True, ref=0, neighbour=False) -> L.NccMotionParameters:
+    p = L.NccMotionParameters()
+    p.patch_size, p.motion_grid_spacing = patch, spacing
+    p.motion_grid_dims[:] = grid
+    p.search_margins[:] = margins
+    p.abs_cor_threshold, p.rel_cor_threshold, p.min_patch_variance = abs_thr, rel_thr, min_var
+    p.use_subpixel, p.reference_frame, p.neighbour_compare = subpixel, ref, neighbour
+    p.frame_count = frames
+    p.image_dims[:] = (W, H)
+    p.data_type = L.TYPE_F32
+    return p
+
+
 # ---- C API over GL buffers --------------------------------------------------
 def _uints(v):
     arr = (C.c_uint * len(v))(*v)

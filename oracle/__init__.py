@@ -62,6 +62,9 @@ def load() -> C.CDLL:
     lib.or_das_constants.argtypes = [C.c_void_p, C.POINTER(DasConsts)]
     lib.or_das.restype = C.c_int
     lib.or_das.argtypes = [DP, C.POINTER(DasConsts), C.c_int, C.c_int, DP, C.POINTER(C.c_longlong)]
+    lib.or_zncc_valid.argtypes = [DP, C.c_int, C.c_int, C.c_int, DP, C.c_int, C.c_int, C.c_int, DP]
+    lib.or_motion_detection.restype = C.c_int
+    lib.or_motion_detection.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.POINTER(C.c_float)]
     _lib = lib
     return lib
 
@@ -168,3 +171,24 @@ def pipeline(acq, cubic: bool = True, cf_hercules: bool = False, return_stages: 
     if return_stages:
         return dict(conv=conv, dec=dec, iq=iq, vol=vol, active=act)
     return vol
+
+
+def zncc_valid(src: np.ndarray, tpl: np.ndarray) -> np.ndarray:
+    """Valid-region ZNCC map [Hs-Ht+1][Ws-Wt+1] (NPP CrossCorrValid_NormLevel semantics)."""
+    src = np.ascontiguousarray(src, np.float64)
+    tpl = np.ascontiguousarray(tpl, np.float64)
+    sh, sw = src.shape
+    th, tw = tpl.shape
+    out = np.zeros((sh - th + 1, sw - tw + 1))
+    load().or_zncc_valid(_dp(src), sw, sh, sw, _dp(tpl), tw, th, tw, _dp(out))
+    return out
+
+
+def motion_detection(images: np.ndarray, params) -> np.ndarray:
+    """images f32 [F][H][W] -> float4 maps [F][gy][gx][4] (reference block matching)."""
+    images = np.ascontiguousarray(images, np.float32)
+    F = params.frame_count
+    out = np.zeros((F, params.motion_grid_dims[1], params.motion_grid_dims[0], 4), np.float32)
+    load().or_motion_detection(images.ctypes.data_as(C.POINTER(C.c_float)), C.addressof(params),
+                               out.ctypes.data_as(C.POINTER(C.c_float)))
+    return out

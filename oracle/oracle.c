@@ -494,3 +494,185 @@ int or_das(const double* iq, const or_das_consts* k, int interp, int cf_mode, do
     if (active) *active = act_total;
     return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* 2-D block matching.                                                       */
+/* ------------------------------------------------------------------------ */
+
+/* Zero-mean normalised cross-correlation over the valid region
+ * (nppiCrossCorrValid_NormLevel, called at block_match_kernels.cu:214-217):
+ * out[v][u] = sum (T - mean T)(S_uv - mean S_uv) / sqrt(sum (T-mT)^2 sum (S_uv-mS_uv)^2),
+ * with the window S_uv = S[v .. v+th-1][u .. u+tw-1].  A zero denominator gives 0. */
+void or_zncc_valid(const double* src, int sw, int sh, int sstride,
+                   const double* tpl, int tw, int th, int tstride, double* out)
+{
+    const int vw = sw - tw + 1, vh = sh - th + 1;
+    const double n = (double)tw * th;
+    double tm = 0.0;
+    for (int y = 0; y < th; ++y)
+        for (int x = 0; x < tw; ++x) tm += tpl[(size_t)y * tstride + x];
+    tm /= n;
+    double te = 0.0;
+    for (int y = 0; y < th; ++y)
+        for (int x = 0; x < tw; ++x) {
+            double d = tpl[(size_t)y * tstride + x] - tm;
+            te += d * d;
+        }
+    for (int v = 0; v < vh; ++v)
+        for (int u = 0; u < vw; ++u) {
+            double sm = 0.0;
+            for (int y = 0; y < th; ++y)
+                for (int x = 0; x < tw; ++x) sm += src[(size_t)(v + y) * sstride + u + x];
+            sm /= n;
+            double cross = 0.0, se = 0.0;
+            for (int y = 0; y < th; ++y)
+                for (int x = 0; x < tw; ++x) {
+                    double s = src[(size_t)(v + y) * sstride + u + x] - sm;
+                    double t = tpl[(size_t)y * tstride + x] - tm;
+                    cross += s * t;
+                    se += s * s;
+                }
+            double den = sqrt(te * se);
+            out[(size_t)v * vw + u] = den > 0.0 ? cross / den : 0.0;
+        }
+}
+
+/* 3x3 least-squares quadratic fit matrix (block_match_kernels.cuh:130-154):
+ * rows give a (x^2), b (y^2), c (xy), d (x), e (y), f (1) of
+ * a x^2 + b y^2 + c xy + d x + e y + f fitted to the 3x3 neighbourhood
+ * (row-major, x fastest, offsets -1..1). */
+static const double P3[6][9] = {
+    {1.0 / 6, -1.0 / 3, 1.0 / 6, 1.0 / 6, -1.0 / 3, 1.0 / 6, 1.0 / 6, -1.0 / 3, 1.0 / 6},
+    {1.0 / 6, 1.0 / 6, 1.0 / 6, -1.0 / 3, -1.0 / 3, -1.0 / 3, 1.0 / 6, 1.0 / 6, 1.0 / 6},
+    {0.25, 0.0, -0.25, 0.0, 0.0, 0.0, -0.25, 0.0, 0.25},
+    {-1.0 / 6, 0.0, 1.0 / 6, -1.0 / 6, 0.0, 1.0 / 6, -1.0 / 6, 0.0, 1.0 / 6},
+    {-1.0 / 6, -1.0 / 6, -1.0 / 6, 0.0, 0.0, 0.0, 1.0 / 6, 1.0 / 6, 1.0 / 6},
+    {-1.0 / 9, 2.0 / 9, -1.0 / 9, 2.0 / 9, 5.0 / 9, 2.0 / 9, -1.0 / 9, 2.0 / 9, -1.0 / 9},
+};
+
+/* One grid point: block_match_pipeline (block_match_kernels.cu:190-240) =
+ * ZNCC map -> find_peaks_kernel (:7-64: the first 2048 map entries, missing
+ * ones as -1, sorted descending with ties in index order, top 32) ->
+ * test_peaks (:67-185) including its 32-lane shuffle-down max tree. */
+static void block_match_one(const double* tpl_img, const double* src_img, int W, int H,
+                             const NCCMotionParameters* p, int gi, int gj, float* out4)
+{
+    const int P = (int)p->patch_size, sp = (int)p->motion_grid_spacing;
+    const int mx = (int)p->search_margins[0], my = (int)p->search_margins[1];
+    const int half = (int)(p->patch_size / 2u);
+    int tl = gj * sp - half, tt = gi * sp - half;
+    int tr = tl + P - 1, tb = tt + P - 1;
+    int sl = tl - mx, sr = tr + mx, st = tt - my, sbm = tb + my;
+    if (tl < 0) tl = 0;
+    if (tr > W - 1) tr = W - 1;
+    if (sl < 0) sl = 0;
+    if (sr > W - 1) sr = W - 1;
+    if (tt < 0) tt = 0;
+    if (tb > H - 1) tb = H - 1;
+    if (st < 0) st = 0;
+    if (sbm > H - 1) sbm = H - 1;
+    out4[0] = out4[1] = out4[2] = out4[3] = 0.0f;
+    const int tw = tr - tl + 1, th = tb - tt + 1, sw = sr - sl + 1, sh = sbm - st + 1;
+    if (tw <= 0 || th <= 0 || sw < tw || sh < th) return; /* NPP rejects the ROI: entry stays 0 */
+    const int vw = sw - tw + 1, vh = sh - th + 1;
+    double* map = (double*)malloc((size_t)vw * vh * sizeof(double));
+    or_zncc_valid(src_img + (size_t)st * W + sl, sw, sh, W, tpl_img + (size_t)tt * W + tl, tw, th, W, map);
+    const int nsx = tl - sl, nsy = tt - st;
+    const int total = vw * vh;
+    /* top 32 of 2048 virtual entries, stable descending */
+    int idx[32];
+    double val[32];
+    char* taken = (char*)calloc(2048, 1);
+    for (int r = 0; r < 32; ++r) {
+        int best = -1;
+        double bv = 0.0;
+        for (int i = 0; i < 2048; ++i) {
+            if (taken[i]) continue;
+            double v = i < total ? map[i] : -1.0;
+            if (best < 0 || v > bv) { best = i; bv = v; }
+        }
+        taken[best] = 1;
+        idx[r] = best;
+        val[r] = bv;
+    }
+    free(taken);
+    const double no_shift_peak = map[(size_t)nsy * vw + nsx];
+    double pk[32], ox[32], oy[32];
+    for (int c = 0; c < 32; ++c) {
+        const int px = idx[c] % vw, py = idx[c] / vw;
+        double v9[9];
+        int k = 0;
+        for (int y = -1; y <= 1; ++y)
+            for (int x = -1; x <= 1; ++x, ++k) {
+                int yy = py + y, xx = px + x;
+                v9[k] = (yy < 0 || yy >= vh || xx < 0 || xx >= vw) ? 0.0 : map[(size_t)yy * vw + xx];
+            }
+        double co[6];
+        for (int i = 0; i < 6; ++i) {
+            co[i] = 0.0;
+            for (int j = 0; j < 9; ++j) co[i] += P3[i][j] * v9[j];
+        }
+        const double root = sqrt(co[0] * co[0] + co[1] * co[1] + co[2] * co[2] - 2 * co[0] * co[1]);
+        const double s0 = co[0] + co[1] + root, s1 = co[0] + co[1] - root;
+        const double min_sharp = fmin(fabs(s0), fabs(s1));
+        const double den = co[2] * co[2] - 4 * co[1] * co[0];
+        double sx = 0.0, sy = 0.0;
+        if (den != 0.0) {
+            sx = (2 * co[1] * co[3] - co[2] * co[4]) / den;
+            sy = (2 * co[0] * co[4] - co[2] * co[3]) / den;
+        }
+        sx = fmin(fmax(sx, -1.0), 1.0) * (p->use_subpixel ? 1.0 : 0.0);
+        sy = fmin(fmax(sy, -1.0), 1.0) * (p->use_subpixel ? 1.0 : 0.0);
+        ox[c] = sx + px - nsx;
+        oy[c] = sy + py - nsy;
+        const double thr = no_shift_peak * (1.0 + p->rel_cor_threshold);
+        double peak = val[c];
+        if (min_sharp < p->min_patch_variance || s0 >= 0.0 || peak > 1.0 || peak < thr) peak = -2.0;
+        pk[c] = peak;
+    }
+    /* __shfl_down_sync tree over 32 lanes: lanes past the end keep their own value */
+    for (int off = 16; off > 0; off >>= 1) {
+        double npk[32], nox[32], noy[32];
+        for (int l = 0; l < 32; ++l) {
+            const int s = l + off < 32 ? l + off : l;
+            if (pk[s] > pk[l]) { npk[l] = pk[s]; nox[l] = ox[s]; noy[l] = oy[s]; }
+            else { npk[l] = pk[l]; nox[l] = ox[l]; noy[l] = oy[l]; }
+        }
+        memcpy(pk, npk, sizeof pk); memcpy(ox, nox, sizeof ox); memcpy(oy, noy, sizeof oy);
+    }
+    double peak = pk[0], dx = ox[0], dy = oy[0];
+    const double thr = no_shift_peak * (1.0 + p->rel_cor_threshold);
+    if (peak < p->abs_cor_threshold || peak <= no_shift_peak) {
+        dx = dy = 0.0;
+        if (peak < thr) peak = -1.0;
+    }
+    out4[0] = (float)dx;
+    out4[1] = (float)dy;
+    out4[2] = (float)peak;
+    out4[3] = (float)no_shift_peak;
+    free(map);
+}
+
+/* ImageProcessor::ncc_block_match (image_processor.cpp:10-56) over f32 frames
+ * [frame][H][W]; maps: float4 [frame][grid_y][grid_x], the reference frame's zero. */
+int or_motion_detection(const float* images, const NCCMotionParameters* p, float* maps)
+{
+    const int W = (int)p->image_dims[0], H = (int)p->image_dims[1], F = (int)p->frame_count;
+    const int gx = (int)p->motion_grid_dims[0], gy = (int)p->motion_grid_dims[1];
+    memset(maps, 0, sizeof(float) * 4 * (size_t)gx * gy * F);
+    double* imgs = (double*)malloc(sizeof(double) * (size_t)W * H * F);
+    for (size_t i = 0; i < (size_t)W * H * F; ++i) imgs[i] = images[i];
+    for (int f = 0; f < F; ++f) {
+        int ref = (int)p->reference_frame;
+        if (f == ref) continue;
+        if (p->neighbour_compare) ref = ref > f ? f + 1 : f - 1;
+        const double* tpl = imgs + (size_t)ref * W * H;
+        const double* src = imgs + (size_t)f * W * H;
+#pragma omp parallel for collapse(2) schedule(dynamic, 4)
+        for (int gi = 0; gi < gy; ++gi)
+            for (int gj = 0; gj < gx; ++gj)
+                block_match_one(tpl, src, W, H, p, gi, gj, maps + 4 * (((size_t)f * gy + gi) * gx + gj));
+    }
+    free(imgs);
+    return 0;
+}

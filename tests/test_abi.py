@@ -38,7 +38,7 @@ def test_header_declarations_are_exported():
     out = subprocess.run(["nm", "-D", "--defined-only", str(L.LIB_PATH)], capture_output=True, text=True).stdout
     names = {line.split()[-1] for line in out.splitlines() if line.strip()}
     inc = Path(__file__).resolve().parent.parent / "include"
-    pending = {"motion_detect_f32_matlab", "corr_images_f32_matlab", "motion_detection", "corr_images"}
+    pending = set()
     for h in inc.glob("*.h"):
         for m in re.finditer(r"EXPORT\w*\s+[\w\s\*]+?\b(\w+)\s*\(", h.read_text()):
             if m.group(1) in pending:
@@ -74,6 +74,34 @@ def test_beamform_rejects_invalid_parameters(lib, case):
     }[case]
     out_elems = 10 if case == "out_short" else None
     assert _expect_false(acq, mut, out_elems) is False
+
+
+def test_motion_detection_rejects_invalid_parameters(lib):
+    imgs = np.zeros((3, 40, 50), np.float32)
+    maps = np.zeros((3, 4, 5, 4), np.float32)
+    ok = api.motion_params(50, 40, 3, (5, 4), 8, 8, (4, 4))
+    bad = [
+        dict(frame_count=0), dict(reference_frame=3), dict(patch_size=0),
+        dict(search_margins=(100, 100)),  # more offsets than one workgroup holds
+    ]
+    for b in bad:
+        p = api.motion_params(50, 40, 3, (5, 4), 8, 8, (4, 4))
+        for k, v in b.items():
+            if isinstance(v, tuple):
+                getattr(p, k)[:] = v
+            else:
+                setattr(p, k, v)
+        assert api.motion_detection(imgs, maps, p) is False, b
+    assert api.motion_detection(imgs[:1], maps, ok) is False           # image span too small
+    assert api.motion_detection(imgs, maps[:1], ok) is False           # map span too small
+
+
+def test_corr_images_rejects_invalid_dims(lib):
+    t = np.zeros((5, 5), np.float32)
+    s = np.zeros((4, 9), np.float32)
+    assert api.corr_images(t, s, np.zeros((1, 5), np.float32)) is False  # template taller than source
+    assert api.corr_images(np.zeros((0, 3), np.float32), s, np.zeros((1, 1), np.float32)) is False
+    assert api.corr_images(np.zeros((2, 3), np.float32), s, np.zeros((1, 1), np.float32)) is False  # map too small
 
 
 def test_c_api_call_order_errors(lib):
