@@ -165,6 +165,8 @@ def load() -> C.CDLL:
     _proto(lib, "cut_rf_hilbert", B, [V, C.c_int, V])
     _proto(lib, "cut_das_device", B, [V, V, P(CudaBeamformerParameters)])
     _proto(lib, "cut_das_active_taps", B, [P(CudaBeamformerParameters), P(C.c_ulonglong)])
+    _proto(lib, "cut_ncc3d_search", B, [V, V, C.c_int, C.c_int, C.c_int, C.c_int, V, C.c_int, V, C.c_int, V, V])
+    _proto(lib, "cut_envelope", B, [V, C.c_size_t, V])
     _proto(lib, "cut_gl_register_device_buffers", B, [P(V), P(C.c_size_t), u, V, C.c_size_t])
     _proto(lib, "cut_synth_rf", B, [P(CutSynthSpec), P(C.c_int16), P(C.c_int16)])
     _lib = lib
@@ -181,5 +183,6 @@ def exported_symbols() -> list[str]:
         "cut_set_stream", "cut_synchronize", "cut_version", "cut_set_decode_mode", "cut_set_das_options",
         "cut_beamform_device", "cut_rf_init", "cut_rf_set_channel_mapping", "cut_rf_set_match_filter",
         "cut_rf_convert_decode", "cut_rf_hilbert", "cut_das_device", "cut_das_active_taps", "cut_gl_register_device_buffers",
+        "cut_ncc3d_search", "cut_envelope",
         "cut_synth_rf",
     ]

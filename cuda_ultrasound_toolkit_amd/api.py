@@ -159,3 +159,22 @@ def rf_hilbert(d_in, in_stride: int, d_out) -> bool:
 
 def das_device(d_iq, d_volume, bp: L.CudaBeamformerParameters) -> bool:
     return bool(L.load().cut_das_device(d_iq.data_ptr(), d_volume.data_ptr(), C.byref(bp)))
+
+
+def _ptr(t):
+    return None if t is None else t.data_ptr()
+
+
+def ncc3d_search(d_tpl, d_src, stride, d_disp, d_peak, d_block_ids=None, n_blocks=None, d_centre=None,
+                 radius=2) -> bool:
+    """3-D ZNCC block search on device volumes [nz][ny][nx] (16^3 blocks, |d| <= 8)."""
+    nz, ny, nx = d_tpl.shape
+    if n_blocks is None:
+        n_blocks = ((nx - 16) // stride + 1) * ((ny - 16) // stride + 1) * ((nz - 16) // stride + 1)
+    return bool(L.load().cut_ncc3d_search(d_tpl.data_ptr(), d_src.data_ptr(), nx, ny, nz, stride,
+                                          _ptr(d_block_ids), n_blocks, _ptr(d_centre), radius,
+                                          d_disp.data_ptr(), d_peak.data_ptr()))
+
+
+def envelope(d_iq, d_out) -> bool:
+    return bool(L.load().cut_envelope(d_iq.data_ptr(), d_out.numel(), d_out.data_ptr()))

@@ -15,7 +15,7 @@
 namespace {
 
 struct NccState {
-    cut::DeviceBuffer images, maps, tpl, src, corr;
+    cut::DeviceBuffer images, maps, tpl, src, corr, i1, i2;
 };
 
 NccState& ncc_state()
@@ -151,4 +151,38 @@ extern "C" void corr_images_f32_matlab(const float* template_image, const float*
     if (!cuda_toolkit::corr_images({template_image, tn}, {source_image, sn}, {corr_map, cw * ch},
                                    params.template_dims, params.source_dims))
         std::cerr << "Error: Correlation failed." << std::endl;
+}
+
+// ---------------------------------------------------------------------------
+// 3-D block matching (extension entry points, device pointers, async)
+// ---------------------------------------------------------------------------
+extern "C" bool cut_ncc3d_search(const float* d_tpl, const float* d_src, int nx, int ny, int nz, int stride,
+                                 const int* d_block_ids, int n_blocks, const int* d_centre, int radius, int* d_disp,
+                                 float* d_peak)
+{
+    try {
+        std::lock_guard<std::mutex> lock(cut::api_mutex());
+        if (!d_tpl || !d_src || !d_disp || !d_peak) CUT_FAIL("ncc3d: null argument");
+        if (nx < 16 || ny < 16 || nz < 16 || stride <= 0) CUT_FAIL("ncc3d: volume smaller than a 16^3 block");
+        const long long total = (long long)((nx - 16) / stride + 1) * ((ny - 16) / stride + 1) * ((nz - 16) / stride + 1);
+        if (!d_block_ids && n_blocks != total) CUT_FAIL("ncc3d: " << n_blocks << " blocks given, the grid has " << total);
+        if (n_blocks < 0 || n_blocks > total) CUT_FAIL("ncc3d: bad block count");
+        auto& s = ncc_state();
+        const size_t ib = (size_t)(nx + 1) * (ny + 1) * (nz + 1) * sizeof(double);
+        if (!s.i1.ensure(std::max(ib, s.i1.bytes)) || !s.i2.ensure(std::max(ib, s.i2.bytes))) return false;
+        hipStream_t st = cut::stream();
+        if (!cut::ncc3d_integrals(d_src, nx, ny, nz, s.i1.as<double>(), s.i2.as<double>(), st)) return false;
+        return cut::launch_ncc3d(d_tpl, d_src, s.i1.as<double>(), s.i2.as<double>(), nx, ny, nz, stride, d_block_ids,
+                                 n_blocks, d_centre, radius, d_disp, d_peak, st);
+    } catch (...) {
+        std::cerr << "cuda_toolkit: cut_ncc3d_search: exception" << std::endl;
+        return false;
+    }
+}
+
+extern "C" bool cut_envelope(const void* d_iq, size_t count, float* d_out)
+{
+    if (!d_iq || !d_out) CUT_FAIL("envelope: null argument");
+    std::lock_guard<std::mutex> lock(cut::api_mutex());
+    return cut::launch_envelope((const float2*)d_iq, count, d_out, cut::stream());
 }

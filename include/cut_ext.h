@@ -65,6 +65,21 @@ CUT_EXT_EXPORT bool cut_das_device(const void* d_iq, void* d_volume, const CudaB
  * sequence's threshold: the work one DAS launch does (host computation). */
 CUT_EXT_EXPORT bool cut_das_active_taps(const CudaBeamformerParameters* bp, unsigned long long* taps);
 
+/* ---- 3-D block matching of envelope volumes ----------------------------- */
+/* ZNCC of 16^3 template blocks (at `stride`, raster order x fastest) of d_tpl
+ * against d_src ([nz][ny][nx] f32) over shifts |d| <= 8 that keep the source
+ * block inside the volume.  d_centre == NULL: full search; otherwise only
+ * |d - centre[b]| <= radius (radius <= 3, refinement).  d_block_ids (may be
+ * NULL = all blocks in order) selects n_blocks blocks; outputs d_disp int
+ * [n_blocks][3] (dx, dy, dz) and d_peak f32 [n_blocks].  The 2-D reference
+ * counterpart is ImageProcessor::ncc_block_match
+ * (cuda_toolkit/src/image_processing/image_processor.cpp:10-56). */
+CUT_EXT_EXPORT bool cut_ncc3d_search(const float* d_tpl, const float* d_src, int nx, int ny, int nz, int stride,
+                                     const int* d_block_ids, int n_blocks, const int* d_centre, int radius,
+                                     int* d_disp, float* d_peak);
+/* |IQ| of count complex samples into floats. */
+CUT_EXT_EXPORT bool cut_envelope(const void* d_iq, size_t count, float* d_out);
+
 /* ---- GL seam ----------------------------------------------------------- */
 /* Register plain device buffers in place of GL buffers so the C API of
  * cuda_toolkit_ogl.h can run where no GL context exists (tests, headless

@@ -65,6 +65,12 @@ def load() -> C.CDLL:
     lib.or_zncc_valid.argtypes = [DP, C.c_int, C.c_int, C.c_int, DP, C.c_int, C.c_int, C.c_int, DP]
     lib.or_motion_detection.restype = C.c_int
     lib.or_motion_detection.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.POINTER(C.c_float)]
+    FP = C.POINTER(C.c_float)
+    IP = C.POINTER(C.c_int)
+    lib.or_zncc3d_at.restype = C.c_double
+    lib.or_zncc3d_at.argtypes = [FP, FP] + [C.c_int] * 10
+    lib.or_ncc3d_full.argtypes = [FP, FP] + [C.c_int] * 6 + [IP, FP]
+    lib.or_ncc3d_window.argtypes = [FP, FP] + [C.c_int] * 6 + [IP, C.c_int, IP, FP]
     _lib = lib
     return lib
 
@@ -192,3 +198,35 @@ def motion_detection(images: np.ndarray, params) -> np.ndarray:
     load().or_motion_detection(images.ctypes.data_as(C.POINTER(C.c_float)), C.addressof(params),
                                out.ctypes.data_as(C.POINTER(C.c_float)))
     return out
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def ncc3d_full(tpl: np.ndarray, src: np.ndarray, block=16, stride=8, margin=8, centre=None, radius=2):
+    """Full (or windowed, when centre is given) 3-D ZNCC search: (disp [nb][3] int32 (dx,dy,dz), peak [nb])."""
+    tpl = np.ascontiguousarray(tpl, np.float32)
+    src = np.ascontiguousarray(src, np.float32)
+    nz, ny, nx = tpl.shape
+    nb = ((nx - block) // stride + 1) * ((ny - block) // stride + 1) * ((nz - block) // stride + 1)
+    disp = np.zeros((nb, 3), np.int32)
+    peak = np.zeros(nb, np.float32)
+    if centre is None:
+        load().or_ncc3d_full(_fp(tpl), _fp(src), nx, ny, nz, block, stride, margin, _ip(disp), _fp(peak))
+    else:
+        c = np.ascontiguousarray(centre, np.int32)
+        load().or_ncc3d_window(_fp(tpl), _fp(src), nx, ny, nz, block, stride, margin, _ip(c), radius,
+                               _ip(disp), _fp(peak))
+    return disp, peak
+
+
+def zncc3d_at(tpl, src, bx, by, bz, block, dx, dy, dz):
+    tpl = np.ascontiguousarray(tpl, np.float32)
+    src = np.ascontiguousarray(src, np.float32)
+    nz, ny, nx = tpl.shape
+    return load().or_zncc3d_at(_fp(tpl), _fp(src), nx, ny, nz, bx, by, bz, block, dx, dy, dz)

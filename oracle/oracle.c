@@ -676,3 +676,76 @@ int or_motion_detection(const float* images, const NCCMotionParameters* p, float
     free(imgs);
     return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* 3-D ZNCC block matching.                                                  */
+/* ------------------------------------------------------------------------ */
+double or_zncc3d_at(const float* T, const float* S, int nx, int ny, int nz, int bx, int by, int bz, int B,
+                    int dx, int dy, int dz)
+{
+    const int sx = bx + dx, sy = by + dy, sz = bz + dz;
+    if (sx < 0 || sy < 0 || sz < 0 || sx + B > nx || sy + B > ny || sz + B > nz) return -INFINITY;
+    const double n = (double)B * B * B;
+    double tm = 0.0, sm = 0.0;
+    for (int z = 0; z < B; ++z)
+        for (int y = 0; y < B; ++y)
+            for (int x = 0; x < B; ++x) {
+                tm += T[((size_t)(bz + z) * ny + by + y) * nx + bx + x];
+                sm += S[((size_t)(sz + z) * ny + sy + y) * nx + sx + x];
+            }
+    tm /= n;
+    sm /= n;
+    double c = 0.0, te = 0.0, se = 0.0;
+    for (int z = 0; z < B; ++z)
+        for (int y = 0; y < B; ++y)
+            for (int x = 0; x < B; ++x) {
+                const double t = T[((size_t)(bz + z) * ny + by + y) * nx + bx + x] - tm;
+                const double s = S[((size_t)(sz + z) * ny + sy + y) * nx + sx + x] - sm;
+                c += t * s;
+                te += t * t;
+                se += s * s;
+            }
+    const double d = sqrt(te * se);
+    return d > 0.0 ? c / d : 0.0;
+}
+
+static void ncc3d_search(const float* T, const float* S, int nx, int ny, int nz, int B, int stride, int margin,
+                         const int* centre, int radius, int* disp, float* peak)
+{
+    const int gx = (nx - B) / stride + 1, gy = (ny - B) / stride + 1, gz = (nz - B) / stride + 1;
+    const long long nb = (long long)gx * gy * gz;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long long b = 0; b < nb; ++b) {
+        const int ix = (int)(b % gx), iy = (int)((b / gx) % gy), iz = (int)(b / ((long long)gx * gy));
+        const int bx = ix * stride, by = iy * stride, bz = iz * stride;
+        int c0[3] = {0, 0, 0};
+        int r = margin;
+        if (centre) { c0[0] = centre[3 * b]; c0[1] = centre[3 * b + 1]; c0[2] = centre[3 * b + 2]; r = radius; }
+        double best = -INFINITY;
+        int bd[3] = {0, 0, 0};
+        for (int dz = c0[2] - r; dz <= c0[2] + r; ++dz)
+            for (int dy = c0[1] - r; dy <= c0[1] + r; ++dy)
+                for (int dx = c0[0] - r; dx <= c0[0] + r; ++dx) {
+                    if (dx < -margin || dx > margin || dy < -margin || dy > margin || dz < -margin || dz > margin)
+                        continue;
+                    const double v = or_zncc3d_at(T, S, nx, ny, nz, bx, by, bz, B, dx, dy, dz);
+                    if (v > best) { best = v; bd[0] = dx; bd[1] = dy; bd[2] = dz; }
+                }
+        disp[3 * b] = bd[0];
+        disp[3 * b + 1] = bd[1];
+        disp[3 * b + 2] = bd[2];
+        peak[b] = (float)best;
+    }
+}
+
+void or_ncc3d_full(const float* T, const float* S, int nx, int ny, int nz, int B, int stride, int margin, int* disp,
+                   float* peak)
+{
+    ncc3d_search(T, S, nx, ny, nz, B, stride, margin, NULL, 0, disp, peak);
+}
+
+void or_ncc3d_window(const float* T, const float* S, int nx, int ny, int nz, int B, int stride, int margin,
+                     const int* centre, int radius, int* disp, float* peak)
+{
+    ncc3d_search(T, S, nx, ny, nz, B, stride, margin, centre, radius, disp, peak);
+}

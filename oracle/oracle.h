@@ -69,11 +69,18 @@ void or_zncc_valid(const double* src, int sw, int sh, int sstride,
                    const double* tpl, int tw, int th, int tstride, double* out);
 int or_motion_detection(const float* images, const NCCMotionParameters* p, float* maps);
 
-/* ---- 3-D full-search ZNCC block matching ---- */
-void or_ncc3d_full(const float* tpl_vol, const float* src_vol, int nx, int ny, int nz,
-                   int block, int stride, int margin, int* disp, float* peak);
+/* ---- 3-D full-search ZNCC block matching (extension of the 2-D reference to
+ * volumes: blocks of `block`^3 at `stride`, shifts |d| <= margin per axis,
+ * shifts that leave the volume excluded; volumes [nz][ny][nx]) ---- */
 double or_zncc3d_at(const float* tpl_vol, const float* src_vol, int nx, int ny, int nz,
                     int bx, int by, int bz, int block, int dx, int dy, int dz);
+/* disp: int [n_blocks][3] (dx, dy, dz), peak: float [n_blocks]; blocks in
+ * (bz, by, bx) raster order; the first maximum in (dz, dy, dx) raster order wins. */
+void or_ncc3d_full(const float* tpl_vol, const float* src_vol, int nx, int ny, int nz,
+                   int block, int stride, int margin, int* disp, float* peak);
+/* Same search restricted to |d - centre| <= radius for each block. */
+void or_ncc3d_window(const float* tpl_vol, const float* src_vol, int nx, int ny, int nz,
+                     int block, int stride, int margin, const int* centre, int radius, int* disp, float* peak);
 
 #ifdef __cplusplus
 }
