@@ -9,6 +9,7 @@
 
 #include "../../include/cuda_toolkit.hpp"
 #include "../../include/cuda_toolkit_matlab.h"
+#include "../../include/cut_ext.h"
 #include "ncc.hpp"
 #include "pipeline.hpp"
 
