@@ -71,6 +71,10 @@ def load() -> C.CDLL:
     lib.or_zncc3d_at.argtypes = [FP, FP] + [C.c_int] * 10
     lib.or_ncc3d_full.argtypes = [FP, FP] + [C.c_int] * 6 + [IP, FP]
     lib.or_ncc3d_window.argtypes = [FP, FP] + [C.c_int] * 6 + [IP, C.c_int, IP, FP]
+    lib.or_pnet_param_count.restype = C.c_int
+    lib.or_pnet_forward.argtypes = [DP, DP, C.c_int, DP]
+    lib.or_pnet_loss_grad.restype = C.c_double
+    lib.or_pnet_loss_grad.argtypes = [DP, DP, DP, C.c_int, DP]
     _lib = lib
     return lib
 
@@ -230,3 +234,26 @@ def zncc3d_at(tpl, src, bx, by, bz, block, dx, dy, dz):
     src = np.ascontiguousarray(src, np.float32)
     nz, ny, nx = tpl.shape
     return load().or_zncc3d_at(_fp(tpl), _fp(src), nx, ny, nz, bx, by, bz, block, dx, dy, dz)
+
+
+def pnet_param_count() -> int:
+    return load().or_pnet_param_count()
+
+
+def pnet_forward(params: np.ndarray, x: np.ndarray) -> np.ndarray:
+    """x [B][2][16][16][16] -> [B][3] (FP64 CNN forward)."""
+    p = np.ascontiguousarray(params, np.float64)
+    x = np.ascontiguousarray(x, np.float64)
+    B = x.shape[0]
+    out = np.zeros((B, 3))
+    load().or_pnet_forward(_dp(p), _dp(x), B, _dp(out))
+    return out
+
+
+def pnet_loss_grad(params: np.ndarray, x: np.ndarray, target: np.ndarray):
+    p = np.ascontiguousarray(params, np.float64)
+    x = np.ascontiguousarray(x, np.float64)
+    t = np.ascontiguousarray(target, np.float64)
+    g = np.zeros_like(p)
+    loss = load().or_pnet_loss_grad(_dp(p), _dp(x), _dp(t), x.shape[0], _dp(g))
+    return loss, g

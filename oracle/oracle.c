@@ -749,3 +749,149 @@ void or_ncc3d_window(const float* T, const float* S, int nx, int ny, int nz, int
 {
     ncc3d_search(T, S, nx, ny, nz, B, stride, margin, centre, radius, disp, peak);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Displacement-proposal CNN, FP64 forward and backward.                     */
+/* ------------------------------------------------------------------------ */
+typedef struct { int ci, co, din, stride; } conv_spec;
+static const conv_spec PNET[4] = {{2, 16, 16, 2}, {16, 32, 8, 2}, {32, 32, 4, 1}, {32, 32, 4, 2}};
+
+int or_pnet_param_count(void)
+{
+    int n = 0;
+    for (int l = 0; l < 4; ++l) n += PNET[l].co * PNET[l].ci * 27 + PNET[l].co;
+    return n + 3 * 256 + 3;
+}
+
+static void conv_fwd(const conv_spec* c, const double* w, const double* b, const double* in, double* out)
+{
+    const int D = c->din, Do = D / c->stride, s = c->stride;
+    for (int co = 0; co < c->co; ++co)
+        for (int oz = 0; oz < Do; ++oz)
+            for (int oy = 0; oy < Do; ++oy)
+                for (int ox = 0; ox < Do; ++ox) {
+                    double a = b[co];
+                    for (int ci = 0; ci < c->ci; ++ci)
+                        for (int k = 0; k < 27; ++k) {
+                            const int iz = oz * s + k / 9 - 1, iy = oy * s + (k / 3) % 3 - 1, ix = ox * s + k % 3 - 1;
+                            if (iz < 0 || iy < 0 || ix < 0 || iz >= D || iy >= D || ix >= D) continue;
+                            a += w[(co * c->ci + ci) * 27 + k] * in[((size_t)(ci * D + iz) * D + iy) * D + ix];
+                        }
+                    out[((size_t)(co * Do + oz) * Do + oy) * Do + ox] = a > 0.0 ? a : 0.0;
+                }
+}
+
+/* dout is the gradient w.r.t. the ReLU output; act is that output */
+static void conv_bwd(const conv_spec* c, const double* w, const double* in, const double* act, const double* dout,
+                     double* dw, double* db, double* din)
+{
+    const int D = c->din, Do = D / c->stride, s = c->stride;
+    if (din) memset(din, 0, sizeof(double) * c->ci * D * D * D);
+    for (int co = 0; co < c->co; ++co)
+        for (int o = 0; o < Do * Do * Do; ++o) {
+            const size_t oi = (size_t)co * Do * Do * Do + o;
+            if (!(act[oi] > 0.0)) continue;
+            const double g = dout[oi];
+            const int oz = o / (Do * Do), oy = (o / Do) % Do, ox = o % Do;
+            db[co] += g;
+            for (int ci = 0; ci < c->ci; ++ci)
+                for (int k = 0; k < 27; ++k) {
+                    const int iz = oz * s + k / 9 - 1, iy = oy * s + (k / 3) % 3 - 1, ix = ox * s + k % 3 - 1;
+                    if (iz < 0 || iy < 0 || ix < 0 || iz >= D || iy >= D || ix >= D) continue;
+                    const size_t ii = ((size_t)(ci * D + iz) * D + iy) * D + ix;
+                    dw[(co * c->ci + ci) * 27 + k] += g * in[ii];
+                    if (din) din[ii] += g * w[(co * c->ci + ci) * 27 + k];
+                }
+        }
+}
+
+static size_t act_size(int l) { const int Do = PNET[l].din / PNET[l].stride; return (size_t)PNET[l].co * Do * Do * Do; }
+
+static void pnet_run(const double* P, const double* x, double** acts, double* y)
+{
+    const double* p = P;
+    const double* in = x;
+    for (int l = 0; l < 4; ++l) {
+        const double* w = p;
+        p += PNET[l].co * PNET[l].ci * 27;
+        const double* b = p;
+        p += PNET[l].co;
+        conv_fwd(&PNET[l], w, b, in, acts[l]);
+        in = acts[l];
+    }
+    for (int o = 0; o < 3; ++o) {
+        double a = p[3 * 256 + o];
+        for (int i = 0; i < 256; ++i) a += p[o * 256 + i] * in[i];
+        y[o] = a;
+    }
+}
+
+void or_pnet_forward(const double* P, const double* in, int B, double* out)
+{
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int b = 0; b < B; ++b) {
+        double* acts[4];
+        for (int l = 0; l < 4; ++l) acts[l] = (double*)malloc(sizeof(double) * act_size(l));
+        pnet_run(P, in + (size_t)b * 2 * 4096, acts, out + 3 * b);
+        for (int l = 0; l < 4; ++l) free(acts[l]);
+    }
+}
+
+double or_pnet_loss_grad(const double* P, const double* in, const double* target, int B, double* grad)
+{
+    const int np = or_pnet_param_count();
+    memset(grad, 0, sizeof(double) * np);
+    double loss = 0.0;
+#pragma omp parallel
+    {
+        double* g = (double*)calloc(np, sizeof(double));
+        double* acts[4];
+        double* dacts[4];
+        for (int l = 0; l < 4; ++l) {
+            acts[l] = (double*)malloc(sizeof(double) * act_size(l));
+            dacts[l] = (double*)malloc(sizeof(double) * act_size(l));
+        }
+        double lsum = 0.0;
+#pragma omp for schedule(dynamic, 1)
+        for (int b = 0; b < B; ++b) {
+            const double* x = in + (size_t)b * 2 * 4096;
+            double y[3];
+            pnet_run(P, x, acts, y);
+            /* offsets of each parameter block */
+            size_t off[10], o = 0;
+            for (int l = 0; l < 4; ++l) {
+                off[2 * l] = o; o += PNET[l].co * PNET[l].ci * 27;
+                off[2 * l + 1] = o; o += PNET[l].co;
+            }
+            off[8] = o; off[9] = o + 768;
+            double dy[3];
+            for (int i = 0; i < 3; ++i) {
+                const double e = y[i] - target[3 * b + i];
+                lsum += 0.5 * e * e / B;
+                dy[i] = e / B;
+            }
+            for (int i = 0; i < 3; ++i) {
+                g[off[9] + i] += dy[i];
+                for (int k = 0; k < 256; ++k) g[off[8] + i * 256 + k] += dy[i] * acts[3][k];
+            }
+            for (int k = 0; k < 256; ++k) {
+                double a = 0.0;
+                for (int i = 0; i < 3; ++i) a += dy[i] * P[off[8] + i * 256 + k];
+                dacts[3][k] = a;
+            }
+            for (int l = 3; l >= 0; --l) {
+                const double* lin = l ? acts[l - 1] : x;
+                conv_bwd(&PNET[l], P + off[2 * l], lin, acts[l], dacts[l], g + off[2 * l], g + off[2 * l + 1],
+                         l ? dacts[l - 1] : NULL);
+            }
+        }
+#pragma omp critical
+        {
+            for (int i = 0; i < np; ++i) grad[i] += g[i];
+            loss += lsum;
+        }
+        free(g);
+        for (int l = 0; l < 4; ++l) { free(acts[l]); free(dacts[l]); }
+    }
+    return loss;
+}

@@ -82,6 +82,17 @@ void or_ncc3d_full(const float* tpl_vol, const float* src_vol, int nx, int ny, i
 void or_ncc3d_window(const float* tpl_vol, const float* src_vol, int nx, int ny, int nz,
                      int block, int stride, int margin, const int* centre, int radius, int* disp, float* peak);
 
+/* ---- displacement-proposal CNN (no reference counterpart) ----
+ * Input [B][2][16][16][16] (template, source blocks, each zero-mean unit-std);
+ * L1 conv 2->16 k3 s2 p1 -> 8^3, L2 16->32 s2 -> 4^3, L3 32->32 s1 -> 4^3,
+ * L4 32->32 s2 -> 2^3 (ReLU after each), FC 256 -> 3.  Parameters packed as
+ * w1 b1 w2 b2 w3 b3 w4 b4 wf bf, conv weights [co][ci][kz][ky][kx].
+ * or_pnet_forward: out [B][3].  or_pnet_loss_grad: loss = (1/B) sum 0.5|y-t|^2,
+ * grad of the loss w.r.t. every parameter (same packing). */
+int or_pnet_param_count(void);
+void or_pnet_forward(const double* params, const double* in, int B, double* out);
+double or_pnet_loss_grad(const double* params, const double* in, const double* target, int B, double* grad);
+
 #ifdef __cplusplus
 }
 #endif
