@@ -168,13 +168,15 @@ extern "C" bool cut_ncc3d_search(const float* d_tpl, const float* d_src, int nx,
         const long long total = (long long)((nx - 16) / stride + 1) * ((ny - 16) / stride + 1) * ((nz - 16) / stride + 1);
         if (!d_block_ids && n_blocks != total) CUT_FAIL("ncc3d: " << n_blocks << " blocks given, the grid has " << total);
         if (n_blocks < 0 || n_blocks > total) CUT_FAIL("ncc3d: bad block count");
+        if (d_centre && (radius < 0 || radius > 3)) CUT_FAIL("ncc3d: window radius must be 0..3");
         auto& s = ncc_state();
-        const size_t ib = (size_t)(nx + 1) * (ny + 1) * (nz + 1) * sizeof(double);
-        if (!s.i1.ensure(std::max(ib, s.i1.bytes)) || !s.i2.ensure(std::max(ib, s.i2.bytes))) return false;
+        const size_t wb = cut::ncc3d_boxsum_doubles(nx, ny, nz) * sizeof(double);
+        const size_t tb = cut::ncc3d_boxsum_tmp_doubles(nx, ny, nz) * sizeof(double);
+        if (!s.i1.ensure(std::max(wb, s.i1.bytes)) || !s.i2.ensure(std::max(tb, s.i2.bytes))) return false;
         hipStream_t st = cut::stream();
-        if (!cut::ncc3d_integrals(d_src, nx, ny, nz, s.i1.as<double>(), s.i2.as<double>(), st)) return false;
-        return cut::launch_ncc3d(d_tpl, d_src, s.i1.as<double>(), s.i2.as<double>(), nx, ny, nz, stride, d_block_ids,
-                                 n_blocks, d_centre, radius, d_disp, d_peak, st);
+        if (!cut::ncc3d_boxsums(d_src, nx, ny, nz, s.i1.as<double>(), s.i2.as<double>(), st)) return false;
+        return cut::launch_ncc3d(d_tpl, d_src, s.i1.as<double>(), nx, ny, nz, stride, d_block_ids, n_blocks, nullptr,
+                                 false, d_centre, radius, d_disp, d_peak, st);
     } catch (...) {
         std::cerr << "cuda_toolkit: cut_ncc3d_search: exception" << std::endl;
         return false;

@@ -23,12 +23,18 @@ bool launch_block_match2d(const float* d_images, const NccDev& p, float4* d_maps
 bool launch_corr_images(const double* d_tpl_c, double tpl_energy, int tw, int th, const float* d_src, int sw, int sh,
                         float* d_out, hipStream_t st);
 
-// 3-D search (16^3 blocks, |d| <= 8): integrals first, then full (centre == nullptr)
-// or windowed search over `nblocks` blocks (ids == nullptr: blocks 0..nblocks-1).
-bool ncc3d_integrals(const float* d_src, int nx, int ny, int nz, double* d_i1, double* d_i2, hipStream_t st);
-bool launch_ncc3d(const float* d_tpl, const float* d_src, const double* d_i1, const double* d_i2, int nx, int ny,
-                  int nz, int stride, const int* d_ids, int nblocks, const int* d_centre, int radius, int* d_disp,
-                  float* d_peak, hipStream_t st);
+// 3-D search (16^3 blocks, |d| <= 8).  FP64 box sums of the source first
+// (ncc3d_boxsums), then the full search (centre == nullptr) or the windowed
+// search |d - centre| <= radius over `nblocks` blocks (ids == nullptr: blocks
+// 0..nblocks-1).  count (full search only, may be null): a device block count
+// that bounds the grid.  by_id: results (and window centres) are indexed by
+// block id rather than by list position.
+size_t ncc3d_boxsum_doubles(int nx, int ny, int nz);
+size_t ncc3d_boxsum_tmp_doubles(int nx, int ny, int nz);
+bool ncc3d_boxsums(const float* d_src, int nx, int ny, int nz, double* d_out, double* d_tmp, hipStream_t st);
+bool launch_ncc3d(const float* d_tpl, const float* d_src, const double* d_w, int nx, int ny, int nz, int stride,
+                  const int* d_ids, int nblocks, const int* d_count, bool by_id, const int* d_centre, int radius,
+                  int* d_disp, float* d_peak, hipStream_t st);
 bool launch_envelope(const float2* d_iq, size_t n, float* d_out, hipStream_t st);
 
 }  // namespace cut

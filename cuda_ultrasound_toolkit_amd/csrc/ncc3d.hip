@@ -8,13 +8,19 @@
 // inside the volume; the best shift (first maximum in (dz, dy, dx) raster
 // order) and its ZNCC are returned.
 //
-// Full search: one workgroup per block.  The template minus its mean sits in
-// LDS.  Thread (dy, dz pair) walks the template rows (y, z) and correlates
-// each 16-sample template row with the 32-sample source row it meets, giving
-// all 17 dx shifts at once (16 x 17 FMAs per 32 loaded samples); consecutive z
-// shifts share source rows, so each row is loaded once for two shifts.  Window
-// sums of S and S^2 come from FP64 integral volumes (built once per source
-// volume) and the energy normalisation is done in FP64.
+// Window sums of S and S^2 over every 16^3 box of the source come from FP64
+// box-sum volumes, built once per source volume by three separable passes;
+// the energy normalisation is done in FP64.
+//
+// Full search: one workgroup per block, one thread per (dz, dy) shift pair
+// holding the 17 dx correlations.  The 32^3 source region is streamed through
+// LDS one z-plane at a time (double-buffered); at source plane Z the thread
+// correlates template plane Z - dz with it, each 16-sample template row
+// against the 32-sample source row it meets (16 x 17 FMAs per row pair).
+//
+// Windowed search (|d - centre| <= radius): the (16 + 2r)^3 source region
+// sits in LDS; threads own a (dz, dy) pair and a subset of template planes
+// and hold the 2r+1 dx correlations; partial sums are added in a fixed order.
 #include "common.hpp"
 #include "ncc.hpp"
 
@@ -26,68 +32,78 @@ constexpr int NB = 16;           // block edge
 constexpr int NM = 8;            // search margin
 constexpr int NS = 2 * NM + 1;   // shifts per axis
 constexpr int NROW = NB + 2 * NM;
-constexpr int N3_THREADS = 160;  // 17 dy x 9 dz-pairs = 153 used
+constexpr int NF_THREADS = 320;  // 17 dz x 17 dy = 289 used
+constexpr int NW_THREADS = 256;
+constexpr int RS = 36;           // LDS row stride (floats) of a staged source plane
+constexpr int TPS = 260;         // LDS plane stride (floats) of the template
 
-// Integral volume with a zero border: I[z+1][y+1][x+1] = sum over [0..z][0..y][0..x].
-__global__ void integral_x_kernel(const float* __restrict__ src, int nx, int ny, int nz, double* __restrict__ i1,
-                                  double* __restrict__ i2)
+// ---------------------------------------------------------------------------
+// box sums: W[z][y][x] = sum over [z, z+16) x [y, y+16) x [x, x+16)
+// ---------------------------------------------------------------------------
+__global__ void boxsum_x_kernel(const float* __restrict__ src, int nx, int ny, int nz, double* __restrict__ a1,
+                                double* __restrict__ a2)
 {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;  // (z, y) row
-    const int P = nx + 1, Q = ny + 1;
-    if (r >= (nz + 1) * (ny + 1)) return;
-    const int z = r / Q, y = r % Q;
-    double* o1 = i1 + (size_t)r * P;
-    double* o2 = i2 + (size_t)r * P;
-    o1[0] = 0.0;
-    o2[0] = 0.0;
-    if (z == 0 || y == 0) {
-        for (int x = 1; x < P; ++x) { o1[x] = 0.0; o2[x] = 0.0; }
-        return;
+    const int wx = nx - NB + 1;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)nz * ny * wx) return;
+    const int x = (int)(i % wx);
+    const size_t row = i / wx;
+    const float* s = src + row * nx + x;
+    double p = 0.0, q = 0.0;
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+        const double v = s[k];
+        p += v;
+        q += v * v;
     }
-    const float* s = src + ((size_t)(z - 1) * ny + (y - 1)) * nx;
-    double a = 0.0, b = 0.0;
-    for (int x = 0; x < nx; ++x) {
-        const double v = s[x];
-        a += v;
-        b += v * v;
-        o1[x + 1] = a;
-        o2[x + 1] = b;
-    }
+    a1[i] = p;
+    a2[i] = q;
 }
 
-__global__ void integral_scan_kernel(double* __restrict__ i1, double* __restrict__ i2, int P, int Q, int R, int axis)
+// sum of 16 consecutive entries along an axis of stride `step` (elements)
+__global__ void boxsum_axis_kernel(const double* __restrict__ a1, const double* __restrict__ a2, size_t n_out,
+                                   int inner, int n_in_axis, int n_out_axis, size_t step, double* __restrict__ b1,
+                                   double* __restrict__ b2)
 {
-    // prefix along y (axis 1) or z (axis 2) of a (R)(Q)(P) array
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (axis == 1) {
-        if (t >= R * P) return;
-        const int z = t / P, x = t % P;
-        double a = 0.0, b = 0.0;
-        for (int y = 0; y < Q; ++y) {
-            const size_t i = ((size_t)z * Q + y) * P + x;
-            a += i1[i]; b += i2[i];
-            i1[i] = a; i2[i] = b;
-        }
-    } else {
-        if (t >= Q * P) return;
-        double a = 0.0, b = 0.0;
-        for (int z = 0; z < R; ++z) {
-            const size_t i = (size_t)z * Q * P + t;
-            a += i1[i]; b += i2[i];
-            i1[i] = a; i2[i] = b;
-        }
+    // output index = (outer, k, inner_i) with k < n_out_axis; input (outer, k + j, inner_i)
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_out) return;
+    const size_t in_i = i % inner;
+    const size_t rest = i / inner;
+    const int k = (int)(rest % n_out_axis);
+    const size_t outer = rest / n_out_axis;
+    const size_t base = (outer * n_in_axis + k) * inner + in_i;
+    double p = 0.0, q = 0.0;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        p += a1[base + j * step];
+        q += a2[base + j * step];
     }
+    b1[i] = p;
+    b2[i] = q;
 }
 
-__device__ __forceinline__ double box_sum(const double* __restrict__ I, int P, int Q, int x, int y, int z, int B)
+// d_out: 2 * nz * ny * (nx - 15) doubles, receives [W1 | W2] (each (nz-15)(ny-15)(nx-15));
+// d_tmp: 2 * nz * (ny - 15) * (nx - 15) doubles.
+bool ncc3d_boxsums(const float* d_src, int nx, int ny, int nz, double* d_out, double* d_tmp, hipStream_t st)
 {
-    auto at = [&](int zz, int yy, int xx) { return I[((size_t)zz * Q + yy) * P + xx]; };
-    return at(z + B, y + B, x + B) - at(z, y + B, x + B) - at(z + B, y, x + B) - at(z + B, y + B, x) + at(z, y, x + B) +
-           at(z, y + B, x) + at(z + B, y, x) - at(z, y, x);
+    const int wx = nx - NB + 1, wy = ny - NB + 1, wz = nz - NB + 1;
+    const size_t na = (size_t)nz * ny * wx, nbq = (size_t)nz * wy * wx, nw = (size_t)wz * wy * wx;
+    boxsum_x_kernel<<<(unsigned)((na + 255) / 256), 256, 0, st>>>(d_src, nx, ny, nz, d_out, d_out + na);
+    boxsum_axis_kernel<<<(unsigned)((nbq + 255) / 256), 256, 0, st>>>(d_out, d_out + na, nbq, wx, ny, wy, (size_t)wx,
+                                                                       d_tmp, d_tmp + nbq);
+    boxsum_axis_kernel<<<(unsigned)((nw + 255) / 256), 256, 0, st>>>(d_tmp, d_tmp + nbq, nw, wy * wx, nz, wz,
+                                                                      (size_t)wy * wx, d_out, d_out + nw);
+    CUT_HIP_CHECK(hipGetLastError());
+    return true;
 }
+
+size_t ncc3d_boxsum_doubles(int nx, int ny, int nz) { return 2 * (size_t)nz * ny * (nx - NB + 1); }
+size_t ncc3d_boxsum_tmp_doubles(int nx, int ny, int nz) { return 2 * (size_t)nz * (ny - NB + 1) * (nx - NB + 1); }
 
 struct N3Geom {
     int nx, ny, nz, stride, gx, gy;
+    int wx, wy;  // box-sum volume row length and rows per plane
 };
 
 __device__ __forceinline__ void block_origin(const N3Geom& g, int b, int& bx, int& by, int& bz)
@@ -97,43 +113,44 @@ __device__ __forceinline__ void block_origin(const N3Geom& g, int b, int& bx, in
     bz = (b / (g.gx * g.gy)) * g.stride;
 }
 
-// template minus its mean into LDS; returns the FP64 energy (all threads)
+// template minus its mean into LDS (plane stride TPS); returns the FP64 energy (all threads)
 __device__ double load_template(const float* __restrict__ tpl, const N3Geom& g, int bx, int by, int bz, float* tl,
                                 double* red)
 {
-    const int tid = threadIdx.x, nt = blockDim.x;
+    const int tid = threadIdx.x, nt = blockDim.x, nw = (nt + 63) / 64;
     double s = 0.0;
     for (int k = tid; k < NB * NB * NB; k += nt) {
         const int x = k % NB, y = (k / NB) % NB, z = k / (NB * NB);
         const float v = tpl[((size_t)(bz + z) * g.ny + by + y) * g.nx + bx + x];
-        tl[k] = v;
+        tl[z * TPS + y * NB + x] = v;
         s += v;
     }
     for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
     if ((tid & 63) == 0) red[tid >> 6] = s;
     __syncthreads();
     double tot = 0.0;
-    for (int w = 0; w < (nt + 63) / 64; ++w) tot += red[w];
+    for (int w = 0; w < nw; ++w) tot += red[w];
     const double mean = tot / (NB * NB * NB);
     __syncthreads();
     double e = 0.0;
     for (int k = tid; k < NB * NB * NB; k += nt) {
-        const double d = (double)tl[k] - mean;
-        tl[k] = (float)d;
+        const int i = (k >> 8) * TPS + (k & 255);
+        const double d = (double)tl[i] - mean;
+        tl[i] = (float)d;
         e += d * d;
     }
     for (int o = 32; o > 0; o >>= 1) e += __shfl_down(e, o, 64);
     if ((tid & 63) == 0) red[tid >> 6] = e;
     __syncthreads();
     double te = 0.0;
-    for (int w = 0; w < (nt + 63) / 64; ++w) te += red[w];
+    for (int w = 0; w < nw; ++w) te += red[w];
     __syncthreads();
     return te;
 }
 
 __device__ __forceinline__ bool better(float v, int i, float bv, int bi) { return v > bv || (v == bv && i < bi); }
 
-// workgroup argmax (value, shift index) -> lane 0 of wave 0 holds the result
+// workgroup argmax (value, shift index) -> thread 0 holds the result
 __device__ void block_argmax(float& bv, int& bi, float* rv, int* ri)
 {
     const int tid = threadIdx.x;
@@ -150,181 +167,238 @@ __device__ void block_argmax(float& bv, int& bi, float* rv, int* ri)
     }
 }
 
-__device__ __forceinline__ float zncc_from(float cross, double te, const double* I1, const double* I2, const N3Geom& g,
-                                           int sx, int sy, int sz)
+__device__ __forceinline__ float zncc_from(float cross, double te, const double* W1, const double* W2,
+                                           const N3Geom& g, int sx, int sy, int sz)
 {
-    const int P = g.nx + 1, Q = g.ny + 1;
+    const size_t i = ((size_t)sz * g.wy + sy) * g.wx + sx;
     const double n = (double)NB * NB * NB;
-    const double s1 = box_sum(I1, P, Q, sx, sy, sz, NB);
-    const double s2 = box_sum(I2, P, Q, sx, sy, sz, NB);
+    const double s1 = W1[i], s2 = W2[i];
     const double se = s2 - s1 * s1 / n;
     const double den = sqrt(te * (se > 0.0 ? se : 0.0));
     return den > 0.0 ? (float)((double)cross / den) : 0.0f;
 }
 
-__global__ __launch_bounds__(N3_THREADS) void ncc3d_full_kernel(const float* __restrict__ tpl,
-                                                                const float* __restrict__ src,
-                                                                const double* __restrict__ I1,
-                                                                const double* __restrict__ I2, const N3Geom g,
-                                                                const int* __restrict__ ids, int* __restrict__ disp,
-                                                                float* __restrict__ peak)
+__device__ __forceinline__ void write_result(int o, int bi, float bv, int* disp, float* peak)
 {
-    __shared__ float tl[NB * NB * NB];
-    __shared__ double red[4];
-    __shared__ float rv[4];
-    __shared__ int ri[4];
-    const int b = ids ? ids[blockIdx.x] : (int)blockIdx.x;
-    int bx, by, bz;
-    block_origin(g, b, bx, by, bz);
-    const double te = load_template(tpl, g, bx, by, bz, tl, red);
-
-    const int tid = threadIdx.x;
-    const int dyi = tid % NS, dzp = tid / NS;
-    const int dy = dyi - NM, dz0 = 2 * dzp - NM;
-    // valid shift ranges (source block inside the volume)
-    const int dxlo = max(-NM, -bx), dxhi = min(NM, g.nx - NB - bx);
-    const int dylo = max(-NM, -by), dyhi = min(NM, g.ny - NB - by);
-    const int dzlo = max(-NM, -bz), dzhi = min(NM, g.nz - NB - bz);
-    const bool active = dzp < (NS + 1) / 2 && dy >= dylo && dy <= dyhi && !(dz0 > dzhi) && !(dz0 + 1 < dzlo);
-
-    float acc0[NS], acc1[NS];
-#pragma unroll
-    for (int i = 0; i < NS; ++i) { acc0[i] = 0.0f; acc1[i] = 0.0f; }
-    if (active) {
-        const int Y0 = by + dy;
-        for (int y = 0; y < NB; ++y) {
-            const int Y = Y0 + y;
-            for (int z = 0; z <= NB; ++z) {
-                // source row at Z = bz + z + dz0 (serves dz0 at z and dz0+1 at z-1)
-                const int Z = min(max(bz + z + dz0, 0), g.nz - 1);
-                const float* srow = src + ((size_t)Z * g.ny + Y) * g.nx;
-                float r[NROW];
-#pragma unroll
-                for (int k = 0; k < NROW; ++k) r[k] = srow[min(max(bx - NM + k, 0), g.nx - 1)];
-                if (z < NB) {
-                    const float* t = tl + (z * NB + y) * NB;
-#pragma unroll
-                    for (int x = 0; x < NB; ++x) {
-                        const float tv = t[x];
-#pragma unroll
-                        for (int d = 0; d < NS; ++d) acc0[d] = fmaf(tv, r[x + d], acc0[d]);
-                    }
-                }
-                if (z > 0) {
-                    const float* t = tl + ((z - 1) * NB + y) * NB;
-#pragma unroll
-                    for (int x = 0; x < NB; ++x) {
-                        const float tv = t[x];
-#pragma unroll
-                        for (int d = 0; d < NS; ++d) acc1[d] = fmaf(tv, r[x + d], acc1[d]);
-                    }
-                }
-            }
-        }
-    }
-    // normalise and pick this thread's best
-    float bv = -3.0e38f;
-    int bi = 0x7fffffff;
-    if (active) {
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int dz = dz0 + h;
-            if (dz < dzlo || dz > dzhi) continue;
-#pragma unroll
-            for (int d = 0; d < NS; ++d) {
-                const int dx = d - NM;
-                if (dx < dxlo || dx > dxhi) continue;
-                const float v = zncc_from(h ? acc1[d] : acc0[d], te, I1, I2, g, bx + dx, by + dy, bz + dz);
-                const int idx = ((dz + NM) * NS + dyi) * NS + d;
-                if (better(v, idx, bv, bi)) { bv = v; bi = idx; }
-            }
-        }
-    }
-    block_argmax(bv, bi, rv, ri);
-    if (tid == 0) {
-        const int d = bi % NS, dyy = (bi / NS) % NS, dzz = bi / (NS * NS);
-        disp[3 * blockIdx.x] = d - NM;
-        disp[3 * blockIdx.x + 1] = dyy - NM;
-        disp[3 * blockIdx.x + 2] = dzz - NM;
-        peak[blockIdx.x] = bv;
+    if (bi == 0x7fffffff) {  // no valid shift
+        disp[3 * o] = 0; disp[3 * o + 1] = 0; disp[3 * o + 2] = 0;
+        peak[o] = -INFINITY;
+    } else {
+        disp[3 * o] = bi % NS - NM;
+        disp[3 * o + 1] = (bi / NS) % NS - NM;
+        disp[3 * o + 2] = bi / (NS * NS) - NM;
+        peak[o] = bv;
     }
 }
 
-// Windowed search |d - centre| <= radius (radius <= 3): one thread per shift.
-__global__ __launch_bounds__(N3_THREADS) void ncc3d_window_kernel(const float* __restrict__ tpl,
-                                                                  const float* __restrict__ src,
-                                                                  const double* __restrict__ I1,
-                                                                  const double* __restrict__ I2, const N3Geom g,
-                                                                  const int* __restrict__ ids,
-                                                                  const int* __restrict__ centre, int radius,
-                                                                  int* __restrict__ disp, float* __restrict__ peak)
+// ids: block list (nullptr = blocks 0..); count: device block count that
+// bounds the grid (nullptr = grid size); by_id: write results at the block id
+// instead of the list position.
+__global__ __launch_bounds__(NF_THREADS) void ncc3d_full_kernel(const float* __restrict__ tpl,
+                                                                const float* __restrict__ src,
+                                                                const double* __restrict__ W1,
+                                                                const double* __restrict__ W2, const N3Geom g,
+                                                                const int* __restrict__ ids,
+                                                                const int* __restrict__ count, int by_id,
+                                                                int* __restrict__ disp, float* __restrict__ peak)
 {
-    __shared__ float tl[NB * NB * NB];
-    __shared__ double red[4];
-    __shared__ float rv[4];
-    __shared__ int ri[4];
+    __shared__ __attribute__((aligned(16))) float tl[NB * TPS];
+    __shared__ __attribute__((aligned(16))) float sp[2][NROW * RS];
+    __shared__ double red[NF_THREADS / 64];
+    __shared__ float rv[NF_THREADS / 64];
+    __shared__ int ri[NF_THREADS / 64];
+    if (count && (int)blockIdx.x >= *count) return;
     const int b = ids ? ids[blockIdx.x] : (int)blockIdx.x;
     int bx, by, bz;
     block_origin(g, b, bx, by, bz);
     const double te = load_template(tpl, g, bx, by, bz, tl, red);
-    const int w = 2 * radius + 1;
+
     const int tid = threadIdx.x;
-    const int cx = centre[3 * blockIdx.x], cy = centre[3 * blockIdx.x + 1], cz = centre[3 * blockIdx.x + 2];
-    float bv = -3.0e38f;
+    const int dzi = tid / NS, dyi = tid % NS;
+    const int dy = dyi - NM, dz = dzi - NM;
+    const int dxlo = max(-NM, -bx), dxhi = min(NM, g.nx - NB - bx);
+    const int dylo = max(-NM, -by), dyhi = min(NM, g.ny - NB - by);
+    const int dzlo = max(-NM, -bz), dzhi = min(NM, g.nz - NB - bz);
+    const bool active = tid < NS * NS && dy >= dylo && dy <= dyhi && dz >= dzlo && dz <= dzhi;
+
+    // staging: 1024 samples of a 32 x 32 source plane, clamped to the volume
+    float pre[4];
+    auto fetch = [&](int zr) {
+        const int Z = min(max(bz - NM + zr, 0), g.nz - 1);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int k = tid + i * NF_THREADS;
+            if (k < NROW * NROW) {
+                const int Y = min(max(by - NM + (k >> 5), 0), g.ny - 1);
+                const int X = min(max(bx - NM + (k & 31), 0), g.nx - 1);
+                pre[i] = src[((size_t)Z * g.ny + Y) * g.nx + X];
+            }
+        }
+    };
+    auto put = [&](float* dst) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int k = tid + i * NF_THREADS;
+            if (k < NROW * NROW) dst[(k >> 5) * RS + (k & 31)] = pre[i];
+        }
+    };
+
+    float acc[NS];
+#pragma unroll
+    for (int i = 0; i < NS; ++i) acc[i] = 0.0f;
+    fetch(0);
+    put(sp[0]);
+    __syncthreads();
+    for (int zr = 0; zr < NROW; ++zr) {
+        if (zr + 1 < NROW) fetch(zr + 1);
+        const int z = zr - dzi;  // template plane meeting source plane zr at this dz
+        if (active && z >= 0 && z < NB) {
+            const float* pl = sp[zr & 1];
+            for (int y = 0; y < NB; ++y) {
+                const float4* s4 = reinterpret_cast<const float4*>(pl + (y + dyi) * RS);
+                const float4* t4 = reinterpret_cast<const float4*>(tl + z * TPS + y * NB);
+                float r[NROW], t[NB];
+#pragma unroll
+                for (int k = 0; k < NROW / 4; ++k) {
+                    const float4 v = s4[k];
+                    r[4 * k] = v.x; r[4 * k + 1] = v.y; r[4 * k + 2] = v.z; r[4 * k + 3] = v.w;
+                }
+#pragma unroll
+                for (int k = 0; k < NB / 4; ++k) {
+                    const float4 v = t4[k];
+                    t[4 * k] = v.x; t[4 * k + 1] = v.y; t[4 * k + 2] = v.z; t[4 * k + 3] = v.w;
+                }
+#pragma unroll
+                for (int x = 0; x < NB; ++x)
+#pragma unroll
+                    for (int d = 0; d < NS; ++d) acc[d] = fmaf(t[x], r[x + d], acc[d]);
+            }
+        }
+        if (zr + 1 < NROW) put(sp[(zr + 1) & 1]);
+        __syncthreads();
+    }
+    float bv = -INFINITY;
     int bi = 0x7fffffff;
-    for (int s = tid; s < w * w * w; s += blockDim.x) {
-        const int dx = cx + s % w - radius, dy = cy + (s / w) % w - radius, dz = cz + s / (w * w) - radius;
+    if (active) {
+        const int sy = by + dy, sz = bz + dz;
+#pragma unroll
+        for (int d = 0; d < NS; ++d) {
+            const int dx = d - NM;
+            if (dx < dxlo || dx > dxhi) continue;
+            const float v = zncc_from(acc[d], te, W1, W2, g, bx + dx, sy, sz);
+            const int idx = (dzi * NS + dyi) * NS + d;
+            if (better(v, idx, bv, bi)) { bv = v; bi = idx; }
+        }
+    }
+    block_argmax(bv, bi, rv, ri);
+    if (tid == 0) write_result(by_id ? b : (int)blockIdx.x, bi, bv, disp, peak);
+}
+
+// Windowed search |d - centre| <= R (R <= 3).
+constexpr int NW_MAXR = 3;
+template <int R>
+__global__ __launch_bounds__(NW_THREADS) void ncc3d_window_kernel(const float* __restrict__ tpl,
+                                                                  const float* __restrict__ src,
+                                                                  const double* __restrict__ W1,
+                                                                  const double* __restrict__ W2, const N3Geom g,
+                                                                  const int* __restrict__ ids,
+                                                                  const int* __restrict__ centre, int by_id,
+                                                                  int* __restrict__ disp, float* __restrict__ peak)
+{
+    constexpr int w = 2 * R + 1, e = NB + 2 * R, pairs = w * w;
+    constexpr int Q = (NW_THREADS / pairs) < NB ? (NW_THREADS / pairs) : NB;
+    static_assert(NW_THREADS * w <= e * e * e, "partial sums reuse the region buffer");
+    __shared__ __attribute__((aligned(16))) float tl[NB * TPS];
+    __shared__ float reg[e * e * e];
+    __shared__ double red[NW_THREADS / 64];
+    __shared__ float rv[NW_THREADS / 64];
+    __shared__ int ri[NW_THREADS / 64];
+    const int b = ids ? ids[blockIdx.x] : (int)blockIdx.x;
+    const int o = by_id ? b : (int)blockIdx.x;
+    int bx, by, bz;
+    block_origin(g, b, bx, by, bz);
+    const int tid = threadIdx.x;
+    const int cx = centre[3 * o], cy = centre[3 * o + 1], cz = centre[3 * o + 2];
+    // region sample 0 sits at source (bx + cx - R, by + cy - R, bz + cz - R); clamped reads
+    // only feed shifts that are excluded below
+    const int ox = bx + cx - R, oy = by + cy - R, oz = bz + cz - R;
+    for (int k = tid; k < e * e * e; k += NW_THREADS) {
+        const int X = min(max(ox + k % e, 0), g.nx - 1);
+        const int Y = min(max(oy + (k / e) % e, 0), g.ny - 1);
+        const int Z = min(max(oz + k / (e * e), 0), g.nz - 1);
+        reg[k] = src[((size_t)Z * g.ny + Y) * g.nx + X];
+    }
+    const double te = load_template(tpl, g, bx, by, bz, tl, red);  // synchronises
+
+    // thread -> (dz, dy) pair and a group of template planes
+    const int pr = tid % pairs, q = tid / pairs;
+    const int dyr = pr % w, dzr = pr / w;
+    float acc[w];
+#pragma unroll
+    for (int i = 0; i < w; ++i) acc[i] = 0.0f;
+    if (q < Q) {
+        for (int z = q; z < NB; z += Q)
+            for (int y = 0; y < NB; ++y) {
+                const float* srow = reg + ((z + dzr) * e + (y + dyr)) * e;
+                const float* t = tl + z * TPS + y * NB;
+                float r[NB + 2 * R];
+#pragma unroll
+                for (int k = 0; k < NB + 2 * R; ++k) r[k] = srow[k];
+#pragma unroll
+                for (int x = 0; x < NB; ++x) {
+                    const float tv = t[x];
+#pragma unroll
+                    for (int d = 0; d < w; ++d) acc[d] = fmaf(tv, r[x + d], acc[d]);
+                }
+            }
+    }
+    __syncthreads();  // the region is no longer read: reuse it for the partial sums
+    float* part = reg;
+    if (q < Q) {
+#pragma unroll
+        for (int d = 0; d < w; ++d) part[(q * pairs + pr) * w + d] = acc[d];
+    }
+    __syncthreads();
+    float bv = -INFINITY;
+    int bi = 0x7fffffff;
+    for (int s = tid; s < pairs * w; s += NW_THREADS) {
+        const int dxr = s % w, p2 = s / w;  // p2 = dzr * w + dyr
+        const int dx = cx - R + dxr, dy = cy - R + p2 % w, dz = cz - R + p2 / w;
         if (dx < -NM || dx > NM || dy < -NM || dy > NM || dz < -NM || dz > NM) continue;
         const int sx = bx + dx, sy = by + dy, sz = bz + dz;
         if (sx < 0 || sy < 0 || sz < 0 || sx + NB > g.nx || sy + NB > g.ny || sz + NB > g.nz) continue;
         float c = 0.0f;
-        for (int z = 0; z < NB; ++z)
-            for (int y = 0; y < NB; ++y) {
-                const float* srow = src + ((size_t)(sz + z) * g.ny + sy + y) * g.nx + sx;
-                const float* t = tl + (z * NB + y) * NB;
-#pragma unroll
-                for (int x = 0; x < NB; ++x) c = fmaf(t[x], srow[x], c);
-            }
-        const float v = zncc_from(c, te, I1, I2, g, sx, sy, sz);
+        for (int qq = 0; qq < Q; ++qq) c += part[(qq * pairs + p2) * w + dxr];
+        const float v = zncc_from(c, te, W1, W2, g, sx, sy, sz);
         const int idx = ((dz + NM) * NS + dy + NM) * NS + dx + NM;
         if (better(v, idx, bv, bi)) { bv = v; bi = idx; }
     }
     block_argmax(bv, bi, rv, ri);
-    if (tid == 0) {
-        if (bi == 0x7fffffff) {  // no valid shift in the window
-            disp[3 * blockIdx.x] = cx; disp[3 * blockIdx.x + 1] = cy; disp[3 * blockIdx.x + 2] = cz;
-            peak[blockIdx.x] = -3.0e38f;
-        } else {
-            disp[3 * blockIdx.x] = bi % NS - NM;
-            disp[3 * blockIdx.x + 1] = (bi / NS) % NS - NM;
-            disp[3 * blockIdx.x + 2] = bi / (NS * NS) - NM;
-            peak[blockIdx.x] = bv;
-        }
-    }
+    if (tid == 0) write_result(o, bi, bv, disp, peak);
 }
 
-bool ncc3d_integrals(const float* d_src, int nx, int ny, int nz, double* d_i1, double* d_i2, hipStream_t st)
-{
-    const int P = nx + 1, Q = ny + 1, R = nz + 1;
-    integral_x_kernel<<<(R * Q + 127) / 128, 128, 0, st>>>(d_src, nx, ny, nz, d_i1, d_i2);
-    integral_scan_kernel<<<(R * P + 127) / 128, 128, 0, st>>>(d_i1, d_i2, P, Q, R, 1);
-    integral_scan_kernel<<<(Q * P + 127) / 128, 128, 0, st>>>(d_i1, d_i2, P, Q, R, 2);
-    CUT_HIP_CHECK(hipGetLastError());
-    return true;
-}
-
-bool launch_ncc3d(const float* d_tpl, const float* d_src, const double* d_i1, const double* d_i2, int nx, int ny,
-                  int nz, int stride, const int* d_ids, int nblocks, const int* d_centre, int radius, int* d_disp,
-                  float* d_peak, hipStream_t st)
+bool launch_ncc3d(const float* d_tpl, const float* d_src, const double* d_w, int nx, int ny, int nz, int stride,
+                  const int* d_ids, int nblocks, const int* d_count, bool by_id, const int* d_centre, int radius,
+                  int* d_disp, float* d_peak, hipStream_t st)
 {
     if (nblocks <= 0) return true;
-    N3Geom g{nx, ny, nz, stride, (nx - NB) / stride + 1, (ny - NB) / stride + 1};
+    const int wx = nx - NB + 1, wy = ny - NB + 1, wz = nz - NB + 1;
+    const double* W1 = d_w;
+    const double* W2 = d_w + (size_t)wx * wy * wz;
+    N3Geom g{nx, ny, nz, stride, (nx - NB) / stride + 1, (ny - NB) / stride + 1, wx, wy};
     if (d_centre) {
-        if (radius < 0 || radius > 3) CUT_FAIL("ncc3d: window radius must be 0..3");
-        ncc3d_window_kernel<<<nblocks, N3_THREADS, 0, st>>>(d_tpl, d_src, d_i1, d_i2, g, d_ids, d_centre, radius,
-                                                            d_disp, d_peak);
+        if (radius < 0 || radius > NW_MAXR) CUT_FAIL("ncc3d: window radius must be 0.." << NW_MAXR);
+        if (d_count) CUT_FAIL("ncc3d: device block counts apply to the full search only");
+        const int bid = by_id ? 1 : 0;
+        switch (radius) {
+        case 0: ncc3d_window_kernel<0><<<nblocks, NW_THREADS, 0, st>>>(d_tpl, d_src, W1, W2, g, d_ids, d_centre, bid, d_disp, d_peak); break;
+        case 1: ncc3d_window_kernel<1><<<nblocks, NW_THREADS, 0, st>>>(d_tpl, d_src, W1, W2, g, d_ids, d_centre, bid, d_disp, d_peak); break;
+        case 2: ncc3d_window_kernel<2><<<nblocks, NW_THREADS, 0, st>>>(d_tpl, d_src, W1, W2, g, d_ids, d_centre, bid, d_disp, d_peak); break;
+        default: ncc3d_window_kernel<3><<<nblocks, NW_THREADS, 0, st>>>(d_tpl, d_src, W1, W2, g, d_ids, d_centre, bid, d_disp, d_peak); break;
+        }
     } else {
-        ncc3d_full_kernel<<<nblocks, N3_THREADS, 0, st>>>(d_tpl, d_src, d_i1, d_i2, g, d_ids, d_disp, d_peak);
+        ncc3d_full_kernel<<<nblocks, NF_THREADS, 0, st>>>(d_tpl, d_src, W1, W2, g, d_ids, d_count, by_id ? 1 : 0,
+                                                          d_disp, d_peak);
     }
     CUT_HIP_CHECK(hipGetLastError());
     return true;

@@ -119,8 +119,11 @@ extern "C" bool cut_synth_rf(const CutSynthSpec* sp, int16_t* raw, int16_t* map)
     for (float v : resp) peak = std::max(peak, std::fabs(v));
     if (sp->noise_frac > 0.0f) {
         const double sd = sp->noise_frac * (peak > 0 ? peak : 1.0f);
+        // the noise stream also depends on amp_seed, so the two frames of a
+        // motion pair (same seed, different amp_seed) carry independent noise
+        const uint64_t nseed = sp->seed ^ (sp->amp_seed * 0xA24BAED4963EE407ull);
         parallel_for((long long)T * C, [&](long long row) {
-            uint64_t s = sp->seed ^ (0xD1B54A32D192ED03ull * (uint64_t)(row + 1));
+            uint64_t s = nseed ^ (0xD1B54A32D192ED03ull * (uint64_t)(row + 1));
             float* r = resp.data() + (size_t)row * S;
             for (int i = 0; i < S; ++i) r[i] += (float)(sd * normal(s));
         });
@@ -134,7 +137,31 @@ extern "C" bool cut_synth_rf(const CutSynthSpec* sp, int16_t* raw, int16_t* map)
         std::vector<float> Ht((size_t)T * T);  // Ht[k][t] = H[t][k]
         for (int t = 0; t < T; ++t)
             for (int k = 0; k < T; ++k) Ht[(size_t)k * T + t] = (float)H[(size_t)t * T + k];
-        parallel_for((long long)C, [&](long long c) {
+        // Sylvester order (H[t][k] = (-1)^popcount(t & k)): fast Walsh-Hadamard
+        // transform in double, T log T per sample instead of T^2
+        bool sylvester = (T & (T - 1)) == 0;
+        for (int t = 0; sylvester && t < T; ++t)
+            for (int k = 0; k < T; ++k)
+                if (H[(size_t)t * T + k] != ((__builtin_popcount(t & k) & 1) ? -1 : 1)) {
+                    sylvester = false;
+                    break;
+                }
+        if (sylvester) {
+            parallel_for((long long)C, [&](long long c) {
+                std::vector<double> col(T);
+                for (int s = 0; s < S; ++s) {
+                    for (int t = 0; t < T; ++t) col[t] = resp[(size_t)t * CS + (size_t)c * S + s];
+                    for (int h = 1; h < T; h <<= 1)
+                        for (int i = 0; i < T; i += 2 * h)
+                            for (int j = i; j < i + h; ++j) {
+                                const double a = col[j], b = col[j + h];
+                                col[j] = a + b;
+                                col[j + h] = a - b;
+                            }
+                    for (int k = 0; k < T; ++k) enc[(size_t)k * CS + (size_t)c * S + s] = (float)col[k];
+                }
+            });
+        } else parallel_for((long long)C, [&](long long c) {
             std::vector<float> col(T);
             for (int s = 0; s < S; ++s) {
                 for (int t = 0; t < T; ++t) col[t] = resp[(size_t)t * CS + (size_t)c * S + s];
