@@ -167,6 +167,16 @@ def load() -> C.CDLL:
     _proto(lib, "cut_das_active_taps", B, [P(CudaBeamformerParameters), P(C.c_ulonglong)])
     _proto(lib, "cut_ncc3d_search", B, [V, V, C.c_int, C.c_int, C.c_int, C.c_int, V, C.c_int, V, C.c_int, V, V])
     _proto(lib, "cut_envelope", B, [V, C.c_size_t, V])
+    F = C.c_float
+    _proto(lib, "cut_pnet_param_count", C.c_int, [])
+    _proto(lib, "cut_pnet_set_precision", B, [C.c_int])
+    _proto(lib, "cut_pnet_gather", B, [V, C.c_int, C.c_int, C.c_int, C.c_int, V, C.c_int, V])
+    _proto(lib, "cut_pnet_forward", B, [V, V, C.c_int, V])
+    _proto(lib, "cut_pnet_loss_grad", B, [V, V, V, C.c_int, V, V])
+    _proto(lib, "cut_pnet_adam", B, [V, V, V, V, C.c_int, F, F, F, F, C.c_int, F])
+    _proto(lib, "cut_pnet_infer", B, [V, V, V, C.c_int, C.c_int, C.c_int, C.c_int, V, C.c_int, V, V])
+    _proto(lib, "cut_motion3d_learned", B, [V, V, V, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, F, C.c_int, V, V,
+                                            V, V, P(C.c_int)])
     _proto(lib, "cut_gl_register_device_buffers", B, [P(V), P(C.c_size_t), u, V, C.c_size_t])
     _proto(lib, "cut_synth_rf", B, [P(CutSynthSpec), P(C.c_int16), P(C.c_int16)])
     _lib = lib
@@ -184,5 +194,7 @@ def exported_symbols() -> list[str]:
         "cut_beamform_device", "cut_rf_init", "cut_rf_set_channel_mapping", "cut_rf_set_match_filter",
         "cut_rf_convert_decode", "cut_rf_hilbert", "cut_das_device", "cut_das_active_taps", "cut_gl_register_device_buffers",
         "cut_ncc3d_search", "cut_envelope",
+        "cut_pnet_param_count", "cut_pnet_set_precision", "cut_pnet_gather", "cut_pnet_forward", "cut_pnet_loss_grad", "cut_pnet_adam",
+        "cut_pnet_infer", "cut_motion3d_learned",
         "cut_synth_rf",
     ]

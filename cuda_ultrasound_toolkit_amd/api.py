@@ -178,3 +178,53 @@ def ncc3d_search(d_tpl, d_src, stride, d_disp, d_peak, d_block_ids=None, n_block
 
 def envelope(d_iq, d_out) -> bool:
     return bool(L.load().cut_envelope(d_iq.data_ptr(), d_out.numel(), d_out.data_ptr()))
+
+
+# ---- learned displacement proposal ----------------------------------------
+def pnet_param_count() -> int:
+    return int(L.load().cut_pnet_param_count())
+
+
+def pnet_set_precision(fp32: bool) -> bool:
+    """Training-path products: BF16 MFMA (default) or FP32 (verification)."""
+    return bool(L.load().cut_pnet_set_precision(1 if fp32 else 0))
+
+
+def pnet_gather(d_vols, shape, stride, d_desc, d_out) -> bool:
+    """d_vols: int64 device tensor of volume addresses [2 * pairs] (template,
+    source); d_desc int32 [n][6]; d_out f32 [n][2][16][16][16]."""
+    nz, ny, nx = shape
+    return bool(L.load().cut_pnet_gather(d_vols.data_ptr(), nx, ny, nz, stride, d_desc.data_ptr(), d_desc.shape[0],
+                                         d_out.data_ptr()))
+
+
+def pnet_forward(d_params, d_x, d_y) -> bool:
+    return bool(L.load().cut_pnet_forward(d_params.data_ptr(), d_x.data_ptr(), d_x.shape[0], d_y.data_ptr()))
+
+
+def pnet_loss_grad(d_params, d_x, d_target, d_grad, d_loss=None) -> bool:
+    return bool(L.load().cut_pnet_loss_grad(d_params.data_ptr(), d_x.data_ptr(), d_target.data_ptr(), d_x.shape[0],
+                                            d_grad.data_ptr(), _ptr(d_loss)))
+
+
+def pnet_adam(d_params, d_grad, d_m, d_v, lr, step, beta1=0.9, beta2=0.999, eps=1e-8, wd=0.0) -> bool:
+    return bool(L.load().cut_pnet_adam(d_params.data_ptr(), d_grad.data_ptr(), d_m.data_ptr(), d_v.data_ptr(),
+                                       d_params.numel(), lr, beta1, beta2, eps, step, wd))
+
+
+def pnet_infer(d_params, d_tpl, d_src, stride, d_pred, d_centre=None, d_block_ids=None) -> bool:
+    nz, ny, nx = d_tpl.shape
+    return bool(L.load().cut_pnet_infer(d_params.data_ptr(), d_tpl.data_ptr(), d_src.data_ptr(), nx, ny, nz, stride,
+                                        _ptr(d_block_ids), d_pred.shape[0], d_pred.data_ptr(), _ptr(d_centre)))
+
+
+def motion3d_learned(d_params, d_tpl, d_src, stride, d_disp, d_peak, radius=2, threshold=0.6, boundary_check=True,
+                     d_path=None, d_pred=None, count_fallbacks=False):
+    """Learned 3-D block matching; returns (ok, number of full searches or None)."""
+    nz, ny, nx = d_tpl.shape
+    n = C.c_int(-1)
+    ok = bool(L.load().cut_motion3d_learned(d_params.data_ptr(), d_tpl.data_ptr(), d_src.data_ptr(), nx, ny, nz,
+                                            stride, radius, threshold, int(bool(boundary_check)), d_disp.data_ptr(),
+                                            d_peak.data_ptr(), _ptr(d_path), _ptr(d_pred),
+                                            C.byref(n) if count_fallbacks else None))
+    return ok, (n.value if count_fallbacks else None)

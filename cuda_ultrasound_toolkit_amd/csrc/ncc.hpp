@@ -35,6 +35,11 @@ bool ncc3d_boxsums(const float* d_src, int nx, int ny, int nz, double* d_out, do
 bool launch_ncc3d(const float* d_tpl, const float* d_src, const double* d_w, int nx, int ny, int nz, int stride,
                   const int* d_ids, int nblocks, const int* d_count, bool by_id, const int* d_centre, int radius,
                   int* d_disp, float* d_peak, hipStream_t st);
+// Select the blocks that need the full search after a windowed search (see
+// ncc3d_fallback_kernel); *d_count receives their number, d_ids their ids.
+bool launch_ncc3d_fallback(const int* d_disp, const float* d_peak, const int* d_centre, int nx, int ny, int nz,
+                           int stride, int nb, int radius, float thr, bool boundary, int* d_ids, int* d_count,
+                           int* d_path, hipStream_t st);
 bool launch_envelope(const float2* d_iq, size_t n, float* d_out, hipStream_t st);
 
 }  // namespace cut

@@ -404,6 +404,51 @@ bool launch_ncc3d(const float* d_tpl, const float* d_src, const double* d_w, int
     return true;
 }
 
+// Blocks that need the full search after a windowed search around `centre`:
+// peak below the threshold (or no valid shift), or, with `boundary`, a maximum
+// on a window face that is not a limit of the full search.  Appends them to
+// ids (order unspecified; each block's result does not depend on it).
+__global__ void ncc3d_fallback_kernel(const int* __restrict__ disp, const float* __restrict__ peak,
+                                      const int* __restrict__ centre, int nb, const N3Geom g, int radius, float thr,
+                                      int boundary, int* __restrict__ ids, int* __restrict__ count,
+                                      int* __restrict__ path)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    bool need = !(peak[b] >= thr);
+    if (!need && boundary) {
+        int o[3];
+        block_origin(g, b, o[0], o[1], o[2]);
+        const int n[3] = {g.nx, g.ny, g.nz};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const int d = disp[3 * b + a], c = centre[3 * b + a];
+            const int lo = max(-NM, -o[a]), hi = min(NM, n[a] - NB - o[a]);
+            if ((d == c - radius && d > lo) || (d == c + radius && d < hi)) need = true;
+        }
+    }
+    if (path) path[b] = need ? 1 : 0;
+    if (need) ids[atomicAdd(count, 1)] = b;
+}
+
+__global__ void zero_int_kernel(int* p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+bool launch_ncc3d_fallback(const int* d_disp, const float* d_peak, const int* d_centre, int nx, int ny, int nz,
+                           int stride, int nb, int radius, float thr, bool boundary, int* d_ids, int* d_count,
+                           int* d_path, hipStream_t st)
+{
+    N3Geom g{nx, ny, nz, stride, (nx - NB) / stride + 1, (ny - NB) / stride + 1, nx - NB + 1, ny - NB + 1};
+    zero_int_kernel<<<1, 64, 0, st>>>(d_count, 1);
+    ncc3d_fallback_kernel<<<(nb + 255) / 256, 256, 0, st>>>(d_disp, d_peak, d_centre, nb, g, radius, thr,
+                                                            boundary ? 1 : 0, d_ids, d_count, d_path);
+    CUT_HIP_CHECK(hipGetLastError());
+    return true;
+}
+
 // |IQ| -> float volume
 __global__ void envelope_kernel(const float2* __restrict__ iq, size_t n, float* __restrict__ out)
 {

@@ -80,6 +80,55 @@ CUT_EXT_EXPORT bool cut_ncc3d_search(const float* d_tpl, const float* d_src, int
 /* |IQ| of count complex samples into floats. */
 CUT_EXT_EXPORT bool cut_envelope(const void* d_iq, size_t count, float* d_out);
 
+/* ---- learned displacement proposal (no reference counterpart) ----------
+ * A 3-D CNN over a pair of 16^3 blocks (template block, source block at the
+ * same position, each normalised to zero mean and unit variance) predicts the
+ * block's displacement in voxels.  Layers: conv 2->16 s2, 16->32 s2, 32->32
+ * s1, 32->32 s2 (3^3 kernels, zero border 1, ReLU), FC 256->3.  Parameters are
+ * FP32, packed w1 b1 w2 b2 w3 b3 w4 b4 wf bf, conv weights
+ * [co][ci][kz][ky][kx], FC weights [3][256] over input c*8 + position.  All
+ * pointers are device pointers; calls are asynchronous on the library stream
+ * unless stated. */
+CUT_EXT_EXPORT int cut_pnet_param_count(void);
+/* Normalised block pairs for training: desc[i] = {pair, block, ox, oy, oz,
+ * flips} selects template block `block` (blocks in (z, y, x) raster order at
+ * `stride`) of volume pair `pair` (vols[2 pair], vols[2 pair + 1] are its
+ * template and source volumes, [nz][ny][nx] f32) and the source block at the
+ * same position plus (ox, oy, oz); bits 0/1/2 of flips mirror x/y/z.
+ * out: f32 [n][2][16][16][16]. */
+CUT_EXT_EXPORT bool cut_pnet_gather(const uint64_t* d_vols, int nx, int ny, int nz, int stride, const int* d_desc,
+                                    int n, float* d_out);
+/* Products of the training path (forward and gradient): 0 = BF16 MFMA with
+ * FP32 accumulation (default), 1 = FP32 operands (verification).  The fused
+ * inference always runs BF16 MFMA. */
+CUT_EXT_EXPORT bool cut_pnet_set_precision(int mode);
+/* Training-path forward: x [B][2][16^3] -> y [B][3]. */
+CUT_EXT_EXPORT bool cut_pnet_forward(const float* d_params, const float* d_x, int B, float* d_y);
+/* loss = (1/B) sum_b 0.5 |y_b - t_b|^2 (d_loss: one f32, may be NULL) and its
+ * gradient with respect to every parameter (d_grad: f32 [param_count]). */
+CUT_EXT_EXPORT bool cut_pnet_loss_grad(const float* d_params, const float* d_x, const float* d_target, int B,
+                                       float* d_grad, float* d_loss);
+/* One Adam step (bias-corrected, step counts from 1; wd: decoupled weight decay). */
+CUT_EXT_EXPORT bool cut_pnet_adam(float* d_params, const float* d_grad, float* d_m, float* d_v, int n, float lr,
+                                  float beta1, float beta2, float eps, int step, float wd);
+/* Fused inference for blocks d_ids[0..n) (NULL: blocks 0..n-1) of a volume
+ * pair: d_pred f32 [n][3] (dx, dy, dz); d_centre (may be NULL) int [n][3] =
+ * clamp(rint(pred), -8, 8). */
+CUT_EXT_EXPORT bool cut_pnet_infer(const float* d_params, const float* d_tpl, const float* d_src, int nx, int ny,
+                                   int nz, int stride, const int* d_ids, int n, float* d_pred, int* d_centre);
+/* Learned block matching of every block of the grid: CNN proposal, ZNCC
+ * search within +-radius (<= 3) of the rounded proposal, and the full +-8
+ * search for every block whose refined peak is below `threshold` or (when
+ * boundary_check) whose refined maximum sits on a window face that is not a
+ * limit of the full search.  d_disp int [nb][3], d_peak f32 [nb] as
+ * cut_ncc3d_search; d_path (may be NULL) int [nb]: 1 where the full search
+ * ran; d_pred (may be NULL) f32 [nb][3]: the proposals.  h_fallbacks (may be
+ * NULL): host int receiving the number of full searches (the call then
+ * synchronises). */
+CUT_EXT_EXPORT bool cut_motion3d_learned(const float* d_params, const float* d_tpl, const float* d_src, int nx,
+                                         int ny, int nz, int stride, int radius, float threshold, int boundary_check,
+                                         int* d_disp, float* d_peak, int* d_path, float* d_pred, int* h_fallbacks);
+
 /* ---- GL seam ----------------------------------------------------------- */
 /* Register plain device buffers in place of GL buffers so the C API of
  * cuda_toolkit_ogl.h can run where no GL context exists (tests, headless
