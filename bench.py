@@ -66,6 +66,61 @@ def cpu_baseline(acq, interp):
                        f"stage s/volume decode {t_dec:.2f}, hilbert {t_hil:.2f}, das {t_das:.2f}")}
 
 
+def motion_leg(dev, stream, seed=20261203, repeats=5):
+    """3-D block matching of one 128^3 envelope pair (rigid shift U[-6, 6] voxels,
+    2 % decorrelation): full +-8 ZNCC search vs the learned proposal + +-2 refine."""
+    import numpy as np
+    import torch
+
+    from cuda_ultrasound_toolkit_amd import api, motion_data
+    from cuda_ultrasound_toolkit_amd import motion_learned as ml
+
+    log("bench: motion pair")
+    shift = motion_data.random_shifts(np.random.default_rng(seed), 1)[0]
+    assert api.set_das_options(cubic=False)
+    tpl, src, _ = motion_data.envelope_pair(seed, shift, dev)
+    params, meta = ml.load_params()
+    assert api.pnet_set_input_mode(meta.get("input_mode", ml.INPUT_MODE))
+    dp = torch.from_numpy(params).to(dev)
+    nz, ny, nx = tpl.shape
+    nb = ((nx - 16) // 8 + 1) * ((ny - 16) // 8 + 1) * ((nz - 16) // 8 + 1)
+    fd = torch.zeros((nb, 3), dtype=torch.int32, device=dev)
+    fp = torch.zeros(nb, dtype=torch.float32, device=dev)
+    ld = torch.zeros_like(fd)
+    lp = torch.zeros_like(fp)
+
+    def best_ms(fn):
+        fn()
+        torch.cuda.synchronize()
+        t = []
+        for _ in range(repeats):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            fn()
+            b.record(stream)
+            torch.cuda.synchronize()
+            t.append(a.elapsed_time(b))
+        return float(np.median(t))
+
+    ms_full = best_ms(lambda: api.ncc3d_search(tpl, src, 8, fd, fp))
+    full = fd.cpu().numpy()
+    out = {"workload": f"128^3 envelope pair, shift {np.round(shift, 2).tolist()} voxels, {nb} blocks of 16^3 "
+                       "at stride 8, +-8 search",
+           "full_ms": round(ms_full, 4), "full_blocks_per_s": round(nb / (ms_full * 1e-3)),
+           "full_tflops": round(nb * 2 * 4096 * 17 ** 3 / (ms_full * 1e-3) / 1e12, 2),
+           "confident_fraction": round(float(np.mean(fp.cpu().numpy() >= 0.6)), 4)}
+    thr_default = float(meta.get("threshold", 0.6))
+    for thr in sorted({0.6, thr_default}):
+        _, nf = api.motion3d_learned(dp, tpl, src, 8, ld, lp, threshold=thr, count_fallbacks=True)
+        agree = float(np.mean(np.all(ld.cpu().numpy() == full, axis=1)))
+        ms = best_ms(lambda: api.motion3d_learned(dp, tpl, src, 8, ld, lp, threshold=thr))
+        out[f"learned_thr{thr:.2f}"] = {"ms": round(ms, 4), "blocks_per_s": round(nb / (ms * 1e-3)),
+                                        "speedup_vs_full": round(ms_full / ms, 3), "agreement": round(agree, 4),
+                                        "fallback_rate": round(nf / nb, 4)}
+    log(f"bench: motion {json.dumps(out)}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -74,6 +129,7 @@ def main():
     ap.add_argument("--interp", choices=["linear", "cubic"], default="linear")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--stage-iters", type=int, default=10)
+    ap.add_argument("--no-motion", action="store_true", help="skip the 3-D motion leg (learned vs full search)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one device-to-device copy of the 512 MiB IQ buffer (counter calibration)")
     args = ap.parse_args()
@@ -152,6 +208,7 @@ def main():
         d_copy.copy_(d_iq)
         torch.cuda.synchronize()
         log(f"bench: calibration copy of {d_iq.numel() * 4} bytes")
+    motion = None if args.no_motion else motion_leg(dev, stream)
     api.set_stream(None)
 
     taps = C.c_ulonglong()
@@ -201,6 +258,7 @@ def main():
                             "hilbert_GBs": round(hil_gbs, 1), "hilbert_frac_hbm": round(hil_gbs / PEAK_HBM_GBS, 4),
                             f"das_{other_interp}_TFLOPs": round(das_tflops_other, 3)},
         "cpu_baseline": cpu,
+        "motion": motion,
     }
     print(json.dumps(out), flush=True)
 

@@ -157,6 +157,17 @@ def rf_hilbert(d_in, in_stride: int, d_out) -> bool:
     return bool(L.load().cut_rf_hilbert(d_in.data_ptr(), in_stride, d_out.data_ptr()))
 
 
+def rf_hilbert_format(d_in, in_stride: int, d_out, fmt: int) -> tuple[bool, float]:
+    """Hilbert into IQ stored as fmt (0 complex f32, 1 FP16 x scale, 2 BF16); returns (ok, scale)."""
+    sc = C.c_float(0.0)
+    ok = bool(L.load().cut_rf_hilbert_format(d_in.data_ptr(), in_stride, d_out.data_ptr(), fmt, C.byref(sc)))
+    return ok, sc.value
+
+
+def das_device_format(d_iq, fmt: int, scale: float, d_volume, bp: L.CudaBeamformerParameters) -> bool:
+    return bool(L.load().cut_das_device_format(d_iq.data_ptr(), fmt, scale, d_volume.data_ptr(), C.byref(bp)))
+
+
 def das_device(d_iq, d_volume, bp: L.CudaBeamformerParameters) -> bool:
     return bool(L.load().cut_das_device(d_iq.data_ptr(), d_volume.data_ptr(), C.byref(bp)))
 
@@ -188,6 +199,11 @@ def pnet_param_count() -> int:
 def pnet_set_precision(fp32: bool) -> bool:
     """Training-path products: BF16 MFMA (default) or FP32 (verification)."""
     return bool(L.load().cut_pnet_set_precision(1 if fp32 else 0))
+
+
+def pnet_set_input_mode(mode: int) -> bool:
+    """0: the 16^3 block pair; 1: the pooled 32^3 search regions."""
+    return bool(L.load().cut_pnet_set_input_mode(mode))
 
 
 def pnet_gather(d_vols, shape, stride, d_desc, d_out) -> bool:

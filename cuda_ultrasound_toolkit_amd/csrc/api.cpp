@@ -137,7 +137,7 @@ bool cut_set_decode_mode(int mode)
 
 bool cut_set_das_options(int cubic, int cf_hercules, int iq_format)
 {
-    if (iq_format != 0) CUT_FAIL("IQ format " << iq_format << " not available");
+    if (iq_format < cut::IQ_F32 || iq_format > cut::IQ_BF16) CUT_FAIL("IQ format " << iq_format << " not available");
     std::lock_guard<std::mutex> lock(cut::api_mutex());
     cut::options().das_cubic = cubic ? 1 : 0;
     cut::options().das_cf_hercules = cf_hercules ? 1 : 0;
@@ -190,6 +190,37 @@ bool cut_rf_hilbert(const float* d_in, int in_stride, void* d_out)
     auto& rf = stage_state().rf;
     cut::HilbertInput in{d_in, 0, in_stride, nullptr, 0, (int)rf.C()};
     return rf.hilbert(in, (float2*)d_out, cut::stream());
+}
+
+bool cut_rf_hilbert_format(const float* d_in, int in_stride, void* d_out, int format, float* scale_out)
+{
+    if (!d_in || !d_out) CUT_FAIL("null argument");
+    if (in_stride != 1 && in_stride != 2) CUT_FAIL("input stride must be 1 or 2");
+    if (format < cut::IQ_F32 || format > cut::IQ_BF16) CUT_FAIL("IQ format " << format << " not available");
+    std::lock_guard<std::mutex> lock(cut::api_mutex());
+    auto& rf = stage_state().rf;
+    const float scale = format == cut::IQ_F16 ? rf.fp16_scale() : 1.0f;
+    if (scale_out) *scale_out = scale;
+    cut::HilbertInput in{d_in, 0, in_stride, nullptr, 0, (int)rf.C()};
+    return rf.hilbert(in, d_out, format, scale, cut::stream());
+}
+
+bool cut_das_device_format(const void* d_iq, int format, float scale, void* d_volume,
+                           const CudaBeamformerParameters* bp)
+{
+    return guarded("cut_das_device_format", [&] {
+        if (!d_iq || !d_volume || !bp) CUT_FAIL("null argument");
+        std::lock_guard<std::mutex> lock(cut::api_mutex());
+        cut::DasParams p;
+        if (!cut::das_params_from_bp(*bp, p)) return false;
+        cut::DasOptions o;
+        o.cubic = cut::options().das_cubic;
+        o.cf_hercules = cut::options().das_cf_hercules;
+        auto& ws = stage_state().das_ws;
+        const size_t need = cut::das_workspace_bytes(p);
+        if (need && !ws.ensure(std::max(need, ws.bytes))) return false;
+        return cut::launch_das(d_iq, format, scale, p, o, (float2*)d_volume, ws.ptr, ws.bytes, cut::stream());
+    });
 }
 
 bool cut_das_device(const void* d_iq, void* d_volume, const CudaBeamformerParameters* bp)

@@ -13,7 +13,8 @@ namespace {
 
 struct PnetState {
     cut::PnetTrainer trainer;
-    cut::DeviceBuffer frags, pred, centre, ids, count, boxsum, boxtmp;
+    int input_mode = 1;
+    cut::DeviceBuffer frags, pred, centre, ids, count, boxsum, boxtmp, pool;
 };
 
 PnetState& pnet_state()
@@ -45,6 +46,14 @@ extern "C" bool cut_pnet_set_precision(int mode)
     return true;
 }
 
+extern "C" bool cut_pnet_set_input_mode(int mode)
+{
+    if (mode != 0 && mode != 1) CUT_FAIL("pnet: input mode must be 0 (block) or 1 (pooled search region)");
+    std::lock_guard<std::mutex> lock(cut::api_mutex());
+    pnet_state().input_mode = mode;
+    return true;
+}
+
 extern "C" bool cut_pnet_gather(const uint64_t* d_vols, int nx, int ny, int nz, int stride, const int* d_desc, int n,
                                 float* d_out)
 {
@@ -52,7 +61,7 @@ extern "C" bool cut_pnet_gather(const uint64_t* d_vols, int nx, int ny, int nz, 
         std::lock_guard<std::mutex> lock(cut::api_mutex());
         if (!d_vols || !d_desc || !d_out || n < 0) CUT_FAIL("pnet gather: null argument");
         if (!check_volume(nx, ny, nz, stride)) return false;
-        return cut::pnet_gather(d_vols, nx, ny, nz, stride, d_desc, n, d_out, cut::stream());
+        return cut::pnet_gather(d_vols, nx, ny, nz, stride, d_desc, n, pnet_state().input_mode, d_out, cut::stream());
     } catch (...) {
         std::cerr << "cuda_toolkit: cut_pnet_gather: exception" << std::endl;
         return false;
@@ -107,11 +116,13 @@ extern "C" bool cut_pnet_infer(const float* d_params, const float* d_tpl, const 
         if (!check_volume(nx, ny, nz, stride)) return false;
         if (!d_ids && n > grid_blocks(nx, ny, nz, stride)) CUT_FAIL("pnet infer: more blocks than the grid holds");
         auto& s = pnet_state();
-        if (!s.frags.ensure(cut::PN_FRAG_TOTAL * sizeof(uint16_t))) return false;
+        const size_t pb = 2 * cut::pnet_pool_floats(nx, ny, nz) * sizeof(float);
+        if (!s.frags.ensure(cut::PN_FRAG_TOTAL * sizeof(uint16_t)) || !s.pool.ensure(std::max(pb, s.pool.bytes)))
+            return false;
         hipStream_t st = cut::stream();
         if (!cut::pnet_pack(d_params, s.frags.as<uint16_t>(), st)) return false;
-        return cut::pnet_infer(d_params, s.frags.as<uint16_t>(), d_tpl, d_src, nx, ny, nz, stride, d_ids, n, d_pred,
-                               d_centre, st);
+        return cut::pnet_infer(d_params, s.frags.as<uint16_t>(), d_tpl, d_src, nx, ny, nz, stride, d_ids, n,
+                               s.input_mode, s.pool.as<float>(), d_pred, d_centre, st);
     } catch (...) {
         std::cerr << "cuda_toolkit: cut_pnet_infer: exception" << std::endl;
         return false;
@@ -134,7 +145,8 @@ extern "C" bool cut_motion3d_learned(const float* d_params, const float* d_tpl, 
         if (!s.frags.ensure(cut::PN_FRAG_TOTAL * sizeof(uint16_t)) || !s.pred.ensure((size_t)nb * 3 * sizeof(float)) ||
             !s.centre.ensure((size_t)nb * 3 * sizeof(int)) || !s.ids.ensure((size_t)nb * sizeof(int)) ||
             !s.count.ensure(sizeof(int)) || !s.boxsum.ensure(std::max(wb, s.boxsum.bytes)) ||
-            !s.boxtmp.ensure(std::max(tb, s.boxtmp.bytes)))
+            !s.boxtmp.ensure(std::max(tb, s.boxtmp.bytes)) ||
+            !s.pool.ensure(std::max(2 * cut::pnet_pool_floats(nx, ny, nz) * sizeof(float), s.pool.bytes)))
             return false;
         hipStream_t st = cut::stream();
         float* pred = d_pred ? d_pred : s.pred.as<float>();
@@ -142,8 +154,8 @@ extern "C" bool cut_motion3d_learned(const float* d_params, const float* d_tpl, 
         const double* W = s.boxsum.as<double>();
         if (!cut::pnet_pack(d_params, s.frags.as<uint16_t>(), st) ||
             !cut::ncc3d_boxsums(d_src, nx, ny, nz, s.boxsum.as<double>(), s.boxtmp.as<double>(), st) ||
-            !cut::pnet_infer(d_params, s.frags.as<uint16_t>(), d_tpl, d_src, nx, ny, nz, stride, nullptr, nb, pred,
-                             centre, st) ||
+            !cut::pnet_infer(d_params, s.frags.as<uint16_t>(), d_tpl, d_src, nx, ny, nz, stride, nullptr, nb,
+                             s.input_mode, s.pool.as<float>(), pred, centre, st) ||
             !cut::launch_ncc3d(d_tpl, d_src, W, nx, ny, nz, stride, nullptr, nb, nullptr, false, centre, radius, d_disp,
                                d_peak, st) ||
             !cut::launch_ncc3d_fallback(d_disp, d_peak, centre, nx, ny, nz, stride, nb, radius, threshold,

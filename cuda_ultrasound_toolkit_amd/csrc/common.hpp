@@ -99,8 +99,13 @@ struct HilbertInput {
 };
 bool launch_twiddles(float2* d_tw, int S, hipStream_t st);
 bool launch_filter_spectrum(const float* d_h, int L, int S, float2* d_F, hipStream_t st);
-bool launch_hilbert(const HilbertInput& in, int rows, int S, const float2* d_tw, const float2* d_F,
-                    float2* out, hipStream_t st);
+// Storage formats of IQ (analytic-signal) samples between the Hilbert
+// transform and the DAS: complex f32, half2 holding value * scale (scale a
+// power of two chosen so int16-derived data cannot overflow), bf16x2.
+enum IqFormat { IQ_F32 = 0, IQ_F16 = 1, IQ_BF16 = 2 };
+inline size_t iq_sample_bytes(int fmt) { return fmt == IQ_F32 ? 8 : 4; }
+bool launch_hilbert(const HilbertInput& in, int rows, int S, const float2* d_tw, const float2* d_F, void* out,
+                    int out_fmt, float out_scale, hipStream_t st);
 bool hilbert_supported(int S);
 
 // DAS parameters in kernel form (beamformer.cu:17-98).
@@ -123,7 +128,6 @@ struct DasParams {
 };
 bool das_params_from_bp(const CudaBeamformerParameters& bp, DasParams& p);
 
-// iq_kind: 0 complex f32, 1 half2 (scaled by iq_scale), 2 bf16x2 (scaled).
 struct DasOptions {
     int cubic = 1;         // 1 = Catmull-Rom (reference), 0 = linear
     int cf_hercules = 0;   // apply the coherence factor in HERCULES as well
@@ -132,7 +136,9 @@ struct DasOptions {
 // the partial sums (das_workspace_bytes; 0 when no split is used).
 int das_splits(long long tiles, int gt_total);
 size_t das_workspace_bytes(const DasParams& p);
-bool launch_das(const void* iq, int iq_kind, float iq_scale, const DasParams& p, const DasOptions& o,
+// iq_fmt: an IqFormat; the stored samples are value * iq_scale.  Staged
+// (F16/BF16) IQ is supported by the HERCULES kernel with Hann apodization.
+bool launch_das(const void* iq, int iq_fmt, float iq_scale, const DasParams& p, const DasOptions& o,
                 float2* vol, void* workspace, size_t workspace_bytes, hipStream_t st);
 
 }  // namespace cut

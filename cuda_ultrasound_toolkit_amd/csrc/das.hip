@@ -18,6 +18,8 @@
 // back to direct global reads for that channel.
 #include "common.hpp"
 
+#include <hip/hip_fp16.h>
+
 #include <cmath>
 #include <algorithm>
 #include <type_traits>
@@ -71,24 +73,43 @@ __device__ __forceinline__ float2 cubic2(float2 pm1, float2 p0, float2 p1, float
                        fmaf(h00, p0.y, fmaf(h01, p1.y, fmaf(h10, m0y, h11 * m1y))));
 }
 
-// Read p[i] of a row, index clamped into [0, S-1]: the reference's cubic form
-// touches p[S] at x = S-2 with weight exactly 0, any finite value works.
-__device__ __forceinline__ float2 row_at(const float2* __restrict__ row, int i, int S)
+// IQ sample i of a buffer in format IQK (an IqFormat), as complex f32
+// (staged formats hold value * scale; inv = 1 / scale, a power of two).
+template <int IQK>
+__device__ __forceinline__ float2 iq_at(const void* __restrict__ base, size_t i, float inv)
 {
-    i = min(max(i, 0), S - 1);
-    return row[i];
+    if constexpr (IQK == IQ_F32) {
+        return static_cast<const float2*>(base)[i];
+    } else if constexpr (IQK == IQ_F16) {
+        const float2 f = __half22float2(static_cast<const __half2*>(base)[i]);
+        return make_float2(f.x * inv, f.y * inv);
+    } else {
+        const uint32_t u = static_cast<const uint32_t*>(base)[i];
+        return make_float2(__uint_as_float(u << 16) * inv, __uint_as_float(u & 0xffff0000u) * inv);
+    }
 }
 
-template <bool CUBIC>
-__device__ __forceinline__ float2 interp_global(const float2* __restrict__ row, float x, int S)
+// Read p[i] of a row, index clamped into [0, S-1]: the reference's cubic form
+// touches p[S] at x = S-2 with weight exactly 0, any finite value works.
+template <int IQK>
+__device__ __forceinline__ float2 row_at(const void* __restrict__ base, size_t row, int i, int S, float inv)
+{
+    i = min(max(i, 0), S - 1);
+    return iq_at<IQK>(base, row + i, inv);
+}
+
+template <bool CUBIC, int IQK = IQ_F32>
+__device__ __forceinline__ float2 interp_global(const void* __restrict__ base, size_t row, float x, int S,
+                                                float inv = 1.0f)
 {
     const float fi = floorf(x);
     const float t = x - fi;
     const int i = (int)fi;
     if constexpr (CUBIC)
-        return cubic2(row_at(row, i - 1, S), row_at(row, i, S), row_at(row, i + 1, S), row_at(row, i + 2, S), t);
+        return cubic2(row_at<IQK>(base, row, i - 1, S, inv), row_at<IQK>(base, row, i, S, inv),
+                      row_at<IQK>(base, row, i + 1, S, inv), row_at<IQK>(base, row, i + 2, S, inv), t);
     else
-        return lerp2(row_at(row, i, S), row_at(row, i + 1, S), t);
+        return lerp2(row_at<IQK>(base, row, i, S, inv), row_at<IQK>(base, row, i + 1, S, inv), t);
 }
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
@@ -136,8 +157,8 @@ struct HercGrid {
     int nxt, nyt, nzt, splits, gt_per_split;
 };
 
-template <bool CUBIC, bool FOCUS, bool CF, bool SINAPO>
-__global__ __launch_bounds__(256) void hercules_kernel(const float2* __restrict__ iq, const DasParams p,
+template <bool CUBIC, bool FOCUS, bool CF, bool SINAPO, int IQK>
+__global__ __launch_bounds__(256) void hercules_kernel(const void* __restrict__ iq, float iq_inv, const DasParams p,
                                                        const HercGrid hg, float2* __restrict__ vol,
                                                        float4* __restrict__ part)
 {
@@ -249,7 +270,7 @@ __global__ __launch_bounds__(256) void hercules_kernel(const float2* __restrict_
             }
             const float dy = ey - vy;
             const float dy2 = dy * dy;
-            const float2* trow = iq + (size_t)t * p.C * rowlen;
+            const size_t toff = (size_t)t * p.C * rowlen;
             if (neg) flip();
             float apo_y = 1.0f;
             if constexpr (SINAPO) apo_y = sin_apo_to(ey, p.xdc_max_y, p.to_power);
@@ -276,8 +297,8 @@ __global__ __launch_bounds__(256) void hercules_kernel(const float2* __restrict_
                         const int mode = (len <= HW && len > 0) ? 0 : 1;
                         if (part == 0) { wlo_s[wi] = wlo; wmode_s[wi] = mode; }
                         if (mode == 0) {
-                            const float2* row = trow + (size_t)c * rowlen;
-                            for (int q = part; q < len; q += 16) win[wi * HW + q] = row[wlo + q];
+                            const size_t row = toff + (size_t)c * rowlen + wlo;
+                            for (int q = part; q < len; q += 16) win[wi * HW + q] = iq_at<IQK>(iq, row + q, iq_inv);
                         }
                     }
                 }
@@ -319,13 +340,13 @@ __global__ __launch_bounds__(256) void hercules_kernel(const float2* __restrict_
                             if constexpr (CF) inc[j] = (w != 0.0f) ? fmaf(v.x, v.x, fmaf(v.y, v.y, inc[j])) : inc[j];
                         }
                     } else {
-                        const float2* grow = trow + (size_t)c * rowlen;
+                        const size_t grow = toff + (size_t)c * rowlen;
 #pragma unroll
                         for (int j = 0; j < HT_Z; ++j) {
                             const float w = tap_weight<SINAPO>(rho, kz[j], apo_c);
                             if (w == 0.0f) continue;
                             const float x = tap_index(rho2 + z2[j], p.spm, base[j], hi_clamp);
-                            const float2 v = interp_global<CUBIC>(grow, x, p.S);
+                            const float2 v = interp_global<CUBIC, IQK>(iq, grow, x, p.S, iq_inv);
                             acc[j].x = fmaf(w, v.x, acc[j].x);
                             acc[j].y = fmaf(w, v.y, acc[j].y);
                             if constexpr (CF) inc[j] = fmaf(v.x, v.x, fmaf(v.y, v.y, inc[j]));
@@ -383,7 +404,7 @@ __global__ __launch_bounds__(256) void hercules_reduce(const float4* __restrict_
 // FORCES: one thread per voxel (the volume is y-invariant in the reference's
 // arithmetic; y only selects the output row).
 template <bool CUBIC>
-__global__ __launch_bounds__(256) void forces_kernel(const float2* __restrict__ iq, const DasParams p,
+__global__ __launch_bounds__(256) void forces_kernel(const void* __restrict__ iq, const DasParams p,
                                                      float2* __restrict__ vol)
 {
     const long long nvox = (long long)p.Nx * p.Ny * p.Nz;
@@ -416,7 +437,7 @@ __global__ __launch_bounds__(256) void forces_kernel(const float2* __restrict__ 
                 float apo = rx_apo * tx_apo;
                 float x = fmaf(fsqrt(fmaf(tx, tx, vz2)) + rxd, p.spm, (float)p.delay);
                 x = fminf(fmaxf(x, 1.0f), (float)p.S - 2.0f);
-                const float2 val = interp_global<CUBIC>(iq + ((size_t)t * p.C + c) * p.S, x, p.S);
+                const float2 val = interp_global<CUBIC>(iq, ((size_t)t * p.C + c) * p.S, x, p.S);
                 if (neg) apo = -apo;
                 acc.x = fmaf(apo, val.x, acc.x);
                 acc.y = fmaf(apo, val.y, acc.y);
@@ -447,12 +468,55 @@ int das_splits(long long tiles, int gt_total)
     return (int)std::max<long long>(1, std::min<long long>(s, 16));
 }
 
-bool launch_das(const void* iq, int iq_kind, float iq_scale, const DasParams& p, const DasOptions& o, float2* vol,
+namespace {
+template <bool CU, bool FO, bool CFV, bool SA>
+bool launch_herc_fmt(int fmt, dim3 grid, hipStream_t st, const void* iq, float inv, const DasParams& p,
+                     const HercGrid& hg, float2* vol, float4* part)
+{
+    if (fmt == IQ_F32) {
+        hercules_kernel<CU, FO, CFV, SA, IQ_F32><<<grid, 256, 0, st>>>(iq, inv, p, hg, vol, part);
+        return true;
+    }
+    if constexpr (SA) {
+        CUT_FAIL("DAS: staged IQ formats support Hann apodization only");
+    } else {
+        if (fmt == IQ_F16) hercules_kernel<CU, FO, CFV, SA, IQ_F16><<<grid, 256, 0, st>>>(iq, inv, p, hg, vol, part);
+        else hercules_kernel<CU, FO, CFV, SA, IQ_BF16><<<grid, 256, 0, st>>>(iq, inv, p, hg, vol, part);
+        return true;
+    }
+}
+
+template <bool CU, bool FO, bool CFV>
+bool launch_herc_sa(bool sa, int fmt, dim3 grid, hipStream_t st, const void* iq, float inv, const DasParams& p,
+                    const HercGrid& hg, float2* vol, float4* part)
+{
+    return sa ? launch_herc_fmt<CU, FO, CFV, true>(fmt, grid, st, iq, inv, p, hg, vol, part)
+              : launch_herc_fmt<CU, FO, CFV, false>(fmt, grid, st, iq, inv, p, hg, vol, part);
+}
+
+template <bool CU, bool FO>
+bool launch_herc_cf(bool cf, bool sa, int fmt, dim3 grid, hipStream_t st, const void* iq, float inv,
+                    const DasParams& p, const HercGrid& hg, float2* vol, float4* part)
+{
+    return cf ? launch_herc_sa<CU, FO, true>(sa, fmt, grid, st, iq, inv, p, hg, vol, part)
+              : launch_herc_sa<CU, FO, false>(sa, fmt, grid, st, iq, inv, p, hg, vol, part);
+}
+
+template <bool CU>
+bool launch_herc_fo(bool fo, bool cf, bool sa, int fmt, dim3 grid, hipStream_t st, const void* iq, float inv,
+                    const DasParams& p, const HercGrid& hg, float2* vol, float4* part)
+{
+    return fo ? launch_herc_cf<CU, true>(cf, sa, fmt, grid, st, iq, inv, p, hg, vol, part)
+              : launch_herc_cf<CU, false>(cf, sa, fmt, grid, st, iq, inv, p, hg, vol, part);
+}
+}  // namespace
+
+bool launch_das(const void* iq, int iq_fmt, float iq_scale, const DasParams& p, const DasOptions& o, float2* vol,
                 void* workspace, size_t workspace_bytes, hipStream_t st)
 {
-    (void)iq_scale;
-    if (iq_kind != 0) CUT_FAIL("DAS: staged IQ formats are not enabled in this build");
-    const float2* d = (const float2*)iq;
+    if (iq_fmt != IQ_F32 && iq_fmt != IQ_F16 && iq_fmt != IQ_BF16) CUT_FAIL("DAS: unknown IQ format " << iq_fmt);
+    if (!(iq_scale > 0.0f)) CUT_FAIL("DAS: IQ scale must be positive");
+    const float inv = 1.0f / iq_scale;
     if (p.Nx <= 0 || p.Ny <= 0 || p.Nz <= 0) CUT_FAIL("DAS: empty output grid");
     if (p.sequence == HERCULES) {
         if (p.focus_type != 2 && p.focus_type != 4) CUT_FAIL("Invalid focus for HERCULES.");
@@ -476,25 +540,20 @@ bool launch_das(const void* iq, int iq_kind, float iq_scale, const DasParams& p,
         const bool focus = p.focus_type == 4;
         const bool sinapo = p.apo_type == RX_TO_SIN;
         const bool cf = o.cf_hercules != 0;
-#define CUT_HERC(CU, FO, CFV, SA) hercules_kernel<CU, FO, CFV, SA><<<grid, 256, 0, st>>>(d, p, hg, vol, part)
-#define CUT_HERC_SA(CU, FO, CFV) do { if (sinapo) CUT_HERC(CU, FO, CFV, true); else CUT_HERC(CU, FO, CFV, false); } while (0)
-#define CUT_HERC_CF(CU, FO) do { if (cf) CUT_HERC_SA(CU, FO, true); else CUT_HERC_SA(CU, FO, false); } while (0)
-#define CUT_HERC_FO(CU) do { if (focus) CUT_HERC_CF(CU, true); else CUT_HERC_CF(CU, false); } while (0)
-        if (o.cubic) CUT_HERC_FO(true); else CUT_HERC_FO(false);
-#undef CUT_HERC_FO
-#undef CUT_HERC_CF
-#undef CUT_HERC_SA
-#undef CUT_HERC
+        const bool ok = o.cubic ? launch_herc_fo<true>(focus, cf, sinapo, iq_fmt, grid, st, iq, inv, p, hg, vol, part)
+                                : launch_herc_fo<false>(focus, cf, sinapo, iq_fmt, grid, st, iq, inv, p, hg, vol, part);
+        if (!ok) return false;
         CUT_HIP_CHECK(hipGetLastError());
         if (hg.splits > 1) {
             hercules_reduce<<<(unsigned)((nvox + 255) / 256), 256, 0, st>>>(part, hg.splits, nvox, cf ? 1 : 0, p.cw, vol);
         }
     } else if (p.sequence == FORCES) {
         if (p.readi_groups > 1 && p.encoding != HADAMARD) CUT_FAIL("FORCES only supports Hadamard encoding at this time.");
+        if (iq_fmt != IQ_F32) CUT_FAIL("DAS: staged IQ formats are supported for HERCULES only");
         const long long nvox = (long long)p.Nx * p.Ny * p.Nz;
         dim3 grid((unsigned)((nvox + 255) / 256));
-        if (o.cubic) forces_kernel<true><<<grid, 256, 0, st>>>(d, p, vol);
-        else forces_kernel<false><<<grid, 256, 0, st>>>(d, p, vol);
+        if (o.cubic) forces_kernel<true><<<grid, 256, 0, st>>>(iq, p, vol);
+        else forces_kernel<false><<<grid, 256, 0, st>>>(iq, p, vol);
     } else {
         CUT_FAIL("Beamformer: Unsupported sequence ID " << p.sequence);
     }

@@ -18,6 +18,8 @@
 // Twiddles come from a table generated in FP64.
 #include "common.hpp"
 
+#include <hip/hip_fp16.h>
+
 namespace cut {
 
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
@@ -175,9 +177,17 @@ __device__ __forceinline__ void stockham_pass(int tid, int p, const float2* __re
     }
 }
 
+// round-to-nearest-even FP32 -> BF16 bits
+__device__ __forceinline__ uint32_t bf16_bits(float f)
+{
+    const uint32_t u = __float_as_uint(f);
+    return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
+}
+
 template <int LOGS, int KIND, bool FILTER>
 __global__ __launch_bounds__(256) void hilbert_kernel(HilbertDev in, int rows, const float2* __restrict__ tw,
-                                                      const float2* __restrict__ F, float2* __restrict__ out)
+                                                      const float2* __restrict__ F, void* __restrict__ out,
+                                                      int ofmt, float oscale)
 {
     constexpr int S = 1 << LOGS;
     constexpr int P = S / 16;
@@ -208,8 +218,17 @@ __global__ __launch_bounds__(256) void hilbert_kernel(HilbertDev in, int rows, c
         }
         lds[lds_idx(k)] = v;
     };
-    float2* orow = out + (size_t)rrow * S;
-    auto store_out = [&](int idx, float2 v) { if (active) orow[idx] = v; };
+    const size_t obase = (size_t)rrow * S;
+    auto store_out = [&](int idx, float2 v) {
+        if (!active) return;
+        if (ofmt == IQ_F32) {
+            static_cast<float2*>(out)[obase + idx] = v;
+        } else if (ofmt == IQ_F16) {
+            static_cast<__half2*>(out)[obase + idx] = __floats2half2_rn(v.x * oscale, v.y * oscale);
+        } else {
+            static_cast<uint32_t*>(out)[obase + idx] = bf16_bits(v.x) | (bf16_bits(v.y) << 16);
+        }
+    };
 
     // forward
     int p = 1;
@@ -290,8 +309,8 @@ bool launch_filter_spectrum(const float* d_h, int L, int S, float2* d_F, hipStre
 bool hilbert_supported(int S) { return S >= 16 && S <= 4096 && (S & (S - 1)) == 0; }
 
 template <int LOGS>
-static bool launch_hilbert_s(const HilbertInput& in, int rows, const float2* tw, const float2* F, float2* out,
-                             hipStream_t st)
+static bool launch_hilbert_s(const HilbertInput& in, int rows, const float2* tw, const float2* F, void* out, int ofmt,
+                             float oscale, hipStream_t st)
 {
     constexpr int S = 1 << LOGS;
     constexpr int P = S / 16;
@@ -299,7 +318,7 @@ static bool launch_hilbert_s(const HilbertInput& in, int rows, const float2* tw,
     HilbertDev d{in.ptr, in.stride, in.map, in.R0, in.C};
     dim3 grid((rows + RPW - 1) / RPW);
     dim3 block(P * RPW);
-#define CUT_HIL(K, FL) hilbert_kernel<LOGS, K, FL><<<grid, block, 0, st>>>(d, rows, tw, F, out)
+#define CUT_HIL(K, FL) hilbert_kernel<LOGS, K, FL><<<grid, block, 0, st>>>(d, rows, tw, F, out, ofmt, oscale)
     if (F) {
         if (in.kind == 0) CUT_HIL(0, true); else if (in.kind == 1) CUT_HIL(1, true); else CUT_HIL(2, true);
     } else {
@@ -310,20 +329,21 @@ static bool launch_hilbert_s(const HilbertInput& in, int rows, const float2* tw,
     return true;
 }
 
-bool launch_hilbert(const HilbertInput& in, int rows, int S, const float2* d_tw, const float2* d_F, float2* out,
-                    hipStream_t st)
+bool launch_hilbert(const HilbertInput& in, int rows, int S, const float2* d_tw, const float2* d_F, void* out,
+                    int ofmt, float oscale, hipStream_t st)
 {
     if (rows <= 0) return true;
+    if (ofmt != IQ_F32 && ofmt != IQ_F16 && ofmt != IQ_BF16) CUT_FAIL("Hilbert: unknown output format " << ofmt);
     switch (S) {
-        case 16: return launch_hilbert_s<4>(in, rows, d_tw, d_F, out, st);
-        case 32: return launch_hilbert_s<5>(in, rows, d_tw, d_F, out, st);
-        case 64: return launch_hilbert_s<6>(in, rows, d_tw, d_F, out, st);
-        case 128: return launch_hilbert_s<7>(in, rows, d_tw, d_F, out, st);
-        case 256: return launch_hilbert_s<8>(in, rows, d_tw, d_F, out, st);
-        case 512: return launch_hilbert_s<9>(in, rows, d_tw, d_F, out, st);
-        case 1024: return launch_hilbert_s<10>(in, rows, d_tw, d_F, out, st);
-        case 2048: return launch_hilbert_s<11>(in, rows, d_tw, d_F, out, st);
-        case 4096: return launch_hilbert_s<12>(in, rows, d_tw, d_F, out, st);
+        case 16: return launch_hilbert_s<4>(in, rows, d_tw, d_F, out, ofmt, oscale, st);
+        case 32: return launch_hilbert_s<5>(in, rows, d_tw, d_F, out, ofmt, oscale, st);
+        case 64: return launch_hilbert_s<6>(in, rows, d_tw, d_F, out, ofmt, oscale, st);
+        case 128: return launch_hilbert_s<7>(in, rows, d_tw, d_F, out, ofmt, oscale, st);
+        case 256: return launch_hilbert_s<8>(in, rows, d_tw, d_F, out, ofmt, oscale, st);
+        case 512: return launch_hilbert_s<9>(in, rows, d_tw, d_F, out, ofmt, oscale, st);
+        case 1024: return launch_hilbert_s<10>(in, rows, d_tw, d_F, out, ofmt, oscale, st);
+        case 2048: return launch_hilbert_s<11>(in, rows, d_tw, d_F, out, ofmt, oscale, st);
+        case 4096: return launch_hilbert_s<12>(in, rows, d_tw, d_F, out, ofmt, oscale, st);
         default: CUT_FAIL("Hilbert: sample count " << S << " not supported (powers of two 16..4096)");
     }
 }

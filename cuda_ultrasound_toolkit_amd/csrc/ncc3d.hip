@@ -309,7 +309,7 @@ __global__ __launch_bounds__(NW_THREADS) void ncc3d_window_kernel(const float* _
     constexpr int Q = (NW_THREADS / pairs) < NB ? (NW_THREADS / pairs) : NB;
     static_assert(NW_THREADS * w <= e * e * e, "partial sums reuse the region buffer");
     __shared__ __attribute__((aligned(16))) float tl[NB * TPS];
-    __shared__ float reg[e * e * e];
+    __shared__ __attribute__((aligned(16))) float reg[e * e * e];
     __shared__ double red[NW_THREADS / 64];
     __shared__ float rv[NW_THREADS / 64];
     __shared__ int ri[NW_THREADS / 64];
@@ -341,12 +341,25 @@ __global__ __launch_bounds__(NW_THREADS) void ncc3d_window_kernel(const float* _
             for (int y = 0; y < NB; ++y) {
                 const float* srow = reg + ((z + dzr) * e + (y + dyr)) * e;
                 const float* t = tl + z * TPS + y * NB;
-                float r[NB + 2 * R];
+                float r[NB + 2 * R], tv16[NB];
+                if constexpr (e % 4 == 0) {  // 16-byte aligned rows
 #pragma unroll
-                for (int k = 0; k < NB + 2 * R; ++k) r[k] = srow[k];
+                    for (int k = 0; k < e / 4; ++k) {
+                        const float4 q4 = reinterpret_cast<const float4*>(srow)[k];
+                        r[4 * k] = q4.x; r[4 * k + 1] = q4.y; r[4 * k + 2] = q4.z; r[4 * k + 3] = q4.w;
+                    }
+                } else {
+#pragma unroll
+                    for (int k = 0; k < NB + 2 * R; ++k) r[k] = srow[k];
+                }
+#pragma unroll
+                for (int k = 0; k < NB / 4; ++k) {
+                    const float4 q4 = reinterpret_cast<const float4*>(t)[k];
+                    tv16[4 * k] = q4.x; tv16[4 * k + 1] = q4.y; tv16[4 * k + 2] = q4.z; tv16[4 * k + 3] = q4.w;
+                }
 #pragma unroll
                 for (int x = 0; x < NB; ++x) {
-                    const float tv = t[x];
+                    const float tv = tv16[x];
 #pragma unroll
                     for (int d = 0; d < w; ++d) acc[d] = fmaf(tv, r[x + d], acc[d]);
                 }

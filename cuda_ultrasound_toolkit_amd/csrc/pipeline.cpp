@@ -242,21 +242,34 @@ bool RfPipeline::convert_decode(const void* d_raw, int type, float* d_out, int o
     return launch_convert(d_raw, type, map_.as<int16_t>(), R0_, (int)T_, (int)C_, (int)S_, d_out, out_stride, st);
 }
 
-bool RfPipeline::hilbert(const HilbertInput& in, float2* d_out, hipStream_t st)
+bool RfPipeline::hilbert(const HilbertInput& in, void* d_out, int fmt, float scale, hipStream_t st)
 {
     if (!init_) CUT_FAIL("Session not initialized.");
     if (!hilbert_supported((int)S_)) CUT_FAIL("Hilbert: sample count " << S_ << " is not supported by the FFT.");
     const float2* F = filter_on_ ? filter_spec_.as<float2>() : nullptr;
-    return launch_hilbert(in, (int)(C_ * T_), (int)S_, tw_.as<float2>(), F, d_out, st);
+    return launch_hilbert(in, (int)(C_ * T_), (int)S_, tw_.as<float2>(), F, d_out, fmt, scale, st);
 }
 
-bool RfPipeline::convert_hilbert(const void* d_raw, int type, float2* d_out, hipStream_t st)
+bool RfPipeline::convert_hilbert(const void* d_raw, int type, void* d_out, int fmt, float scale, hipStream_t st)
 {
     if (!init_) CUT_FAIL("Session not initialized.");
     if (!check_map(RC_)) return false;
     if (type != TYPE_I16 && type != TYPE_F32) CUT_FAIL("Data converter: Unsupported input data type.");
     HilbertInput in{d_raw, type == TYPE_I16 ? 1 : 2, 1, map_.as<int16_t>(), (size_t)R0_, (int)C_};
-    return hilbert(in, d_out, st);
+    return hilbert(in, d_out, fmt, scale, st);
+}
+
+float RfPipeline::fp16_scale() const
+{
+    // |decoded| <= 32768; filtering multiplies the bound by sum |h|; the
+    // analytic signal of a length-S row (w_k = 0.75/S, 1.5/S) stays below
+    // 0.75 (1 + (2/pi) ln S + 1) times the filtered bound, under 8 for
+    // S <= 4096.  scale = 2^-k with 32768 * 8 * max(1, sum |h|) * scale <= 32768.
+    double l1 = 0.0;
+    if (filter_on_)
+        for (float v : filter_host_) l1 += std::fabs((double)v);
+    const double gain = 8.0 * std::max(1.0, l1);
+    return (float)std::ldexp(1.0, -(int)std::ceil(std::log2(gain)));
 }
 
 // ---------------------------------------------------------------------------
@@ -452,23 +465,27 @@ bool beamform_device(RfPipeline& rf, BeamformBuffers& bufs, const void* d_raw, s
     DasParams p;
     if (!das_params_from_bp(bp, p)) return false;
 
+    const int fmt = options().iq_format;
+    if (fmt != IQ_F32 && p.sequence != HERCULES) CUT_FAIL("Staged IQ formats are supported for HERCULES only.");
+    if (fmt == IQ_F16 && bp.data_type != TYPE_I16) CUT_FAIL("FP16-staged IQ needs int16 input data.");
+    const float scale = fmt == IQ_F16 ? rf.fp16_scale() : 1.0f;
     const size_t count = (size_t)S * C * T;
-    if (!bufs.iq.ensure(count * sizeof(float2))) return false;
-    float2* iq = bufs.iq.as<float2>();
+    if (!bufs.iq.ensure(count * iq_sample_bytes(fmt))) return false;
+    void* iq = bufs.iq.ptr;
     if (bp.decode != NONE) {
         if (!bufs.dec.ensure(count * sizeof(float))) return false;
         if (!rf.convert_decode(d_raw, bp.data_type, bufs.dec.as<float>(), 1, true, st)) CUT_FAIL("Failed to decode RF data.");
         HilbertInput in{bufs.dec.ptr, 0, 1, nullptr, 0, (int)C};
-        if (!rf.hilbert(in, iq, st)) CUT_FAIL("Failed to apply Hilbert transform.");
+        if (!rf.hilbert(in, iq, fmt, scale, st)) CUT_FAIL("Failed to apply Hilbert transform.");
     } else {
-        if (!rf.convert_hilbert(d_raw, bp.data_type, iq, st)) CUT_FAIL("Failed to apply Hilbert transform.");
+        if (!rf.convert_hilbert(d_raw, bp.data_type, iq, fmt, scale, st)) CUT_FAIL("Failed to apply Hilbert transform.");
     }
     DasOptions o;
     o.cubic = options().das_cubic;
     o.cf_hercules = options().das_cf_hercules;
     const size_t ws = das_workspace_bytes(p);
     if (ws && !bufs.das_ws.ensure(std::max(ws, bufs.das_ws.bytes))) return false;
-    if (!launch_das(iq, 0, 1.0f, p, o, d_vol, bufs.das_ws.ptr, bufs.das_ws.bytes, st)) CUT_FAIL("Beamforming failed.");
+    if (!launch_das(iq, fmt, scale, p, o, d_vol, bufs.das_ws.ptr, bufs.das_ws.bytes, st)) CUT_FAIL("Beamforming failed.");
     return true;
 }
 

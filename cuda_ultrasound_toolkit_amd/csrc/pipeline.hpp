@@ -18,7 +18,7 @@ struct Options {
     int decode_mode = 0;    // 0 auto (int8 MFMA for int16 input), 1 FP32 GEMM, 2 int8 MFMA
     int das_cubic = 1;      // 1 Catmull-Rom (reference), 0 linear
     int das_cf_hercules = 0;
-    int iq_format = 0;      // 0 complex f32
+    int iq_format = 0;      // IqFormat of the IQ handed from the Hilbert transform to the DAS
 };
 Options& options();
 
@@ -34,9 +34,14 @@ public:
     bool set_match_filter(const float* h, size_t length);
     // decode (encoding != NONE) or plain convert into out[(t*C+c)*S+s]*stride
     bool convert_decode(const void* d_raw, int type, float* d_out, int out_stride, bool decode, hipStream_t st);
-    bool hilbert(const HilbertInput& in, float2* d_out, hipStream_t st);
+    bool hilbert(const HilbertInput& in, float2* d_out, hipStream_t st) { return hilbert(in, d_out, IQ_F32, 1.0f, st); }
+    // out in IqFormat fmt, holding value * scale
+    bool hilbert(const HilbertInput& in, void* d_out, int fmt, float scale, hipStream_t st);
     // Hilbert that converts raw input on the fly (decode == NONE fast path)
-    bool convert_hilbert(const void* d_raw, int type, float2* d_out, hipStream_t st);
+    bool convert_hilbert(const void* d_raw, int type, void* d_out, int fmt, float scale, hipStream_t st);
+    // Power-of-two scale for FP16-staged IQ of int16 input: the analytic
+    // signal of int16 data filtered by the current taps stays below 32768.
+    float fp16_scale() const;
     bool check_map(uint raw_channels) const;
 
     uint S() const { return S_; }

@@ -56,24 +56,47 @@ __device__ __forceinline__ float wg_sum256(float v, float* red)
     return (red[0] + red[1]) + (red[2] + red[3]);
 }
 
-// Thread t of 256 loads row (z, y) = (t / 16, t % 16) of a 16^3 block whose
-// first voxel is (x0, y0, z0), with the axes in `flips` mirrored, and
-// normalises it over the block; coordinates are clamped to the volume.
+// Thread t of 256 loads row (z, y) = (t / 16, t % 16) of the network input of
+// a block whose first voxel is (x0, y0, z0), with the axes in `flips`
+// mirrored, and normalises it over the 16^3 input; coordinates are clamped to
+// the volume.  mode 0: the 16^3 block itself; mode 1: the 32^3 neighbourhood
+// [x0 - 8, x0 + 24) (the +-8 search region of the block), averaged 2 x 2 x 2.
 __device__ __forceinline__ void load_norm_row(const float* __restrict__ vol, int nx, int ny, int nz, int x0, int y0,
-                                              int z0, int flips, float v[16], float* red)
+                                              int z0, int flips, int mode, float v[16], float* red)
 {
     const int t = threadIdx.x;
     int z = t >> 4, y = t & 15;
     if (flips & 4) z = 15 - z;
     if (flips & 2) y = 15 - y;
-    const int Z = min(max(z0 + z, 0), nz - 1), Y = min(max(y0 + y, 0), ny - 1);
-    const float* row = vol + ((size_t)Z * ny + Y) * nx;
     float s = 0.0f;
+    if (mode == 0) {
+        const int Z = min(max(z0 + z, 0), nz - 1), Y = min(max(y0 + y, 0), ny - 1);
+        const float* row = vol + ((size_t)Z * ny + Y) * nx;
 #pragma unroll
-    for (int x = 0; x < 16; ++x) {
-        const int xx = (flips & 1) ? 15 - x : x;
-        v[x] = row[min(max(x0 + xx, 0), nx - 1)];
-        s += v[x];
+        for (int x = 0; x < 16; ++x) {
+            const int xx = (flips & 1) ? 15 - x : x;
+            v[x] = row[min(max(x0 + xx, 0), nx - 1)];
+            s += v[x];
+        }
+    } else {
+#pragma unroll
+        for (int x = 0; x < 16; ++x) v[x] = 0.0f;
+        for (int q = 0; q < 4; ++q) {
+            const int Z = min(max(z0 - 8 + 2 * z + (q >> 1), 0), nz - 1);
+            const int Y = min(max(y0 - 8 + 2 * y + (q & 1), 0), ny - 1);
+            const float* row = vol + ((size_t)Z * ny + Y) * nx;
+#pragma unroll
+            for (int x = 0; x < 16; ++x) {
+                const int xx = (flips & 1) ? 15 - x : x;
+                const int X = x0 - 8 + 2 * xx;
+                v[x] += row[min(max(X, 0), nx - 1)] + row[min(max(X + 1, 0), nx - 1)];
+            }
+        }
+#pragma unroll
+        for (int x = 0; x < 16; ++x) {
+            v[x] *= 0.125f;
+            s += v[x];
+        }
     }
     const float mean = wg_sum256(s, red) * (1.0f / 4096.0f);
     float q = 0.0f;
@@ -95,10 +118,62 @@ __device__ __forceinline__ void block_xyz(int b, int gx, int gy, int stride, int
     z = (b / (gx * gy)) * stride;
 }
 
+// 2 x 2 x 2 averages of a volume with clamped coordinates, over the padded
+// grid P[k] = mean of vol[clamp(2k - 8 + {0, 1})] per axis, k < pn = n / 2 + 8:
+// the input of input mode 1 for a block at x0 (x0 even) is P[x0/2 .. x0/2 + 16).
+__global__ void pn_pool_kernel(const float* __restrict__ vol, int nx, int ny, int nz, int pnx, int pny, int pnz,
+                               float* __restrict__ P)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)pnx * pny * pnz) return;
+    const int kx = (int)(i % pnx), ky = (int)((i / pnx) % pny), kz = (int)(i / ((size_t)pnx * pny));
+    float s = 0.0f;
+    for (int q = 0; q < 4; ++q) {
+        const int Z = min(max(2 * kz - 8 + (q >> 1), 0), nz - 1);
+        const int Y = min(max(2 * ky - 8 + (q & 1), 0), ny - 1);
+        const float* row = vol + ((size_t)Z * ny + Y) * nx;
+        s += row[min(max(2 * kx - 8, 0), nx - 1)] + row[min(max(2 * kx - 7, 0), nx - 1)];
+    }
+    P[i] = s * 0.125f;
+}
+
+// Row (z, y) of the 16^3 window of a pooled volume starting at (px, py, pz),
+// normalised over the window.  vec4: rows are 16-byte aligned.
+__device__ __forceinline__ void load_norm_pooled(const float* __restrict__ P, int pnx, int pny, int px, int py, int pz,
+                                                 bool vec4, float v[16], float* red)
+{
+    const int t = threadIdx.x;
+    const float* row = P + ((size_t)(pz + (t >> 4)) * pny + py + (t & 15)) * pnx + px;
+    if (vec4) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float4 q = reinterpret_cast<const float4*>(row)[k];
+            v[4 * k] = q.x; v[4 * k + 1] = q.y; v[4 * k + 2] = q.z; v[4 * k + 3] = q.w;
+        }
+    } else {
+#pragma unroll
+        for (int x = 0; x < 16; ++x) v[x] = row[x];
+    }
+    float s = 0.0f;
+#pragma unroll
+    for (int x = 0; x < 16; ++x) s += v[x];
+    const float mean = wg_sum256(s, red) * (1.0f / 4096.0f);
+    float q = 0.0f;
+#pragma unroll
+    for (int x = 0; x < 16; ++x) {
+        v[x] -= mean;
+        q += v[x] * v[x];
+    }
+    const float var = wg_sum256(q, red) * (1.0f / 4096.0f);
+    const float inv = var > 0.0f ? 1.0f / sqrtf(var) : 0.0f;
+#pragma unroll
+    for (int x = 0; x < 16; ++x) v[x] *= inv;
+}
+
 // desc[i] = {pair, block, ox, oy, oz, flips}: template block `block` of pair
 // `pair`, source block at the same position plus (ox, oy, oz).
 __global__ __launch_bounds__(256) void pn_gather_kernel(const uint64_t* __restrict__ vols, int nx, int ny, int nz,
-                                                        int stride, const int* __restrict__ desc,
+                                                        int stride, const int* __restrict__ desc, int mode,
                                                         float* __restrict__ out)
 {
     __shared__ float red[4];
@@ -110,10 +185,10 @@ __global__ __launch_bounds__(256) void pn_gather_kernel(const uint64_t* __restri
     block_xyz(d[1], gx, gy, stride, bx, by, bz);
     float v[16];
     float* o = out + (size_t)blockIdx.x * 8192 + threadIdx.x * 16;
-    load_norm_row(tpl, nx, ny, nz, bx, by, bz, d[5], v, red);
+    load_norm_row(tpl, nx, ny, nz, bx, by, bz, d[5], mode, v, red);
 #pragma unroll
     for (int x = 0; x < 16; ++x) o[x] = v[x];
-    load_norm_row(src, nx, ny, nz, bx + d[2], by + d[3], bz + d[4], d[5], v, red);
+    load_norm_row(src, nx, ny, nz, bx + d[2], by + d[3], bz + d[4], d[5], mode, v, red);
 #pragma unroll
     for (int x = 0; x < 16; ++x) o[4096 + x] = v[x];
 }
@@ -459,7 +534,9 @@ __global__ __launch_bounds__(256) void pn_infer_kernel(const float* __restrict__
                                                        int nx, int ny, int nz, int stride,
                                                        const int* __restrict__ ids,
                                                        const float* __restrict__ params,
-                                                       const uint16_t* __restrict__ frags,
+                                                       const uint16_t* __restrict__ frags, int mode,
+                                                       const float* __restrict__ ptpl, const float* __restrict__ psrc,
+                                                       int pnx, int pny, int pvec,
                                                        float* __restrict__ pred, int* __restrict__ centre)
 {
     __shared__ __attribute__((aligned(16))) uint32_t xin[4096];  // (template, source) BF16 pairs, [z][y][x]
@@ -477,8 +554,13 @@ __global__ __launch_bounds__(256) void pn_infer_kernel(const float* __restrict__
     // input pair, normalised, as BF16 (template in the low half)
     {
         float v[16], w[16];
-        load_norm_row(tpl, nx, ny, nz, bx, by, bz, 0, v, red);
-        load_norm_row(src, nx, ny, nz, bx, by, bz, 0, w, red);
+        if (ptpl) {
+            load_norm_pooled(ptpl, pnx, pny, bx >> 1, by >> 1, bz >> 1, pvec != 0, v, red);
+            load_norm_pooled(psrc, pnx, pny, bx >> 1, by >> 1, bz >> 1, pvec != 0, w, red);
+        } else {
+            load_norm_row(tpl, nx, ny, nz, bx, by, bz, 0, mode, v, red);
+            load_norm_row(src, nx, ny, nz, bx, by, bz, 0, mode, w, red);
+        }
 #pragma unroll
         for (int x = 0; x < 16; ++x) xin[tid * 16 + x] = (uint32_t)f2bf(v[x]) | ((uint32_t)f2bf(w[x]) << 16);
     }
@@ -721,11 +803,11 @@ bool pnet_adam(float* d_p, const float* d_g, float* d_m, float* d_v, int n, floa
     return true;
 }
 
-bool pnet_gather(const uint64_t* d_vols, int nx, int ny, int nz, int stride, const int* d_desc, int n, float* d_out,
-                 hipStream_t st)
+bool pnet_gather(const uint64_t* d_vols, int nx, int ny, int nz, int stride, const int* d_desc, int n, int mode,
+                 float* d_out, hipStream_t st)
 {
     if (n <= 0) return true;
-    pn_gather_kernel<<<n, 256, 0, st>>>(d_vols, nx, ny, nz, stride, d_desc, d_out);
+    pn_gather_kernel<<<n, 256, 0, st>>>(d_vols, nx, ny, nz, stride, d_desc, mode, d_out);
     CUT_HIP_CHECK(hipGetLastError());
     return true;
 }
@@ -737,11 +819,28 @@ bool pnet_pack(const float* d_params, uint16_t* d_frags, hipStream_t st)
     return true;
 }
 
+size_t pnet_pool_floats(int nx, int ny, int nz) { return (size_t)(nx / 2 + 8) * (ny / 2 + 8) * (nz / 2 + 8); }
+
 bool pnet_infer(const float* d_params, const uint16_t* d_frags, const float* d_tpl, const float* d_src, int nx, int ny,
-                int nz, int stride, const int* d_ids, int n, float* d_pred, int* d_centre, hipStream_t st)
+                int nz, int stride, const int* d_ids, int n, int mode, float* d_pool, float* d_pred, int* d_centre,
+                hipStream_t st)
 {
     if (n <= 0) return true;
-    pn_infer_kernel<<<n, 256, 0, st>>>(d_tpl, d_src, nx, ny, nz, stride, d_ids, d_params, d_frags, d_pred, d_centre);
+    const float* pt = nullptr;
+    const float* ps = nullptr;
+    const int pnx = nx / 2 + 8, pny = ny / 2 + 8, pnz = nz / 2 + 8;
+    if (mode == 1 && d_pool && stride % 2 == 0) {
+        // pool both volumes once; every block then reads a 16^3 window
+        const size_t np = (size_t)pnx * pny * pnz;
+        float* P = d_pool;
+        pn_pool_kernel<<<(unsigned)((np + 255) / 256), 256, 0, st>>>(d_tpl, nx, ny, nz, pnx, pny, pnz, P);
+        pn_pool_kernel<<<(unsigned)((np + 255) / 256), 256, 0, st>>>(d_src, nx, ny, nz, pnx, pny, pnz, P + np);
+        pt = P;
+        ps = P + np;
+    }
+    const int pvec = (pnx % 4 == 0 && stride % 8 == 0) ? 1 : 0;
+    pn_infer_kernel<<<n, 256, 0, st>>>(d_tpl, d_src, nx, ny, nz, stride, d_ids, d_params, d_frags, mode, pt, ps, pnx,
+                                       pny, pvec, d_pred, d_centre);
     CUT_HIP_CHECK(hipGetLastError());
     return true;
 }

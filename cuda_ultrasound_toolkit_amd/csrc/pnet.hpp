@@ -57,10 +57,14 @@ class PnetTrainer {
 
 bool pnet_adam(float* d_p, const float* d_g, float* d_m, float* d_v, int n, float lr, float b1, float b2, float eps,
                int step, float wd, hipStream_t st);
-bool pnet_gather(const uint64_t* d_vols, int nx, int ny, int nz, int stride, const int* d_desc, int n, float* d_out,
-                 hipStream_t st);
+bool pnet_gather(const uint64_t* d_vols, int nx, int ny, int nz, int stride, const int* d_desc, int n, int mode,
+                 float* d_out, hipStream_t st);
 bool pnet_pack(const float* d_params, uint16_t* d_frags, hipStream_t st);
+// Pooled volumes of input mode 1: 2 * pnet_pool_floats floats of scratch (d_pool may be
+// null: the blocks are then pooled on the fly).
+size_t pnet_pool_floats(int nx, int ny, int nz);
 bool pnet_infer(const float* d_params, const uint16_t* d_frags, const float* d_tpl, const float* d_src, int nx, int ny,
-                int nz, int stride, const int* d_ids, int n, float* d_pred, int* d_centre, hipStream_t st);
+                int nz, int stride, const int* d_ids, int n, int mode, float* d_pool, float* d_pred, int* d_centre,
+                hipStream_t st);
 
 }  // namespace cut

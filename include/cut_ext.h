@@ -37,7 +37,11 @@ CUT_EXT_EXPORT const char* cut_version(void);
 CUT_EXT_EXPORT bool cut_set_decode_mode(int mode);
 /* DAS: cubic 1 = Catmull-Rom as the reference, 0 = linear;
  * cf_hercules 1 = apply the coherence factor in HERCULES too (the reference
- * computes it but does not apply it); iq_format 0 = complex f32. */
+ * computes it but does not apply it); iq_format: IQ handed from the Hilbert
+ * transform to the DAS by beamform calls, 0 = complex f32, 1 = FP16 pairs
+ * holding value * 2^-k (k fixed by the match filter so int16 data cannot
+ * overflow; int16 input only), 2 = BF16 pairs.  Staged formats are HERCULES
+ * with Hann apodization; accumulation stays FP32. */
 CUT_EXT_EXPORT bool cut_set_das_options(int cubic, int cf_hercules, int iq_format);
 
 /* ---- whole path on device pointers (async) ------------------------------ */
@@ -60,6 +64,14 @@ CUT_EXT_EXPORT bool cut_rf_hilbert(const float* d_in, int in_stride, void* d_out
 /* DAS of complex IQ [T][C][S] into a complex volume [Nz][Ny][Nx]
  * (cuda_toolkit/src/beamformer/beamformer.cu:139-197). */
 CUT_EXT_EXPORT bool cut_das_device(const void* d_iq, void* d_volume, const CudaBeamformerParameters* bp);
+/* The same two stages with IQ in a given storage format (see
+ * cut_set_das_options): the Hilbert output is written as value * scale
+ * (*scale_out receives the scale used: 2^-k for FP16, 1 otherwise) and the
+ * DAS reads IQ stored that way. */
+CUT_EXT_EXPORT bool cut_rf_hilbert_format(const float* d_in, int in_stride, void* d_out, int format,
+                                          float* scale_out);
+CUT_EXT_EXPORT bool cut_das_device_format(const void* d_iq, int format, float scale, void* d_volume,
+                                          const CudaBeamformerParameters* bp);
 
 /* Number of (voxel, transmit, channel) taps whose apodization passes the
  * sequence's threshold: the work one DAS launch does (host computation). */
@@ -102,6 +114,10 @@ CUT_EXT_EXPORT bool cut_pnet_gather(const uint64_t* d_vols, int nx, int ny, int 
  * FP32 accumulation (default), 1 = FP32 operands (verification).  The fused
  * inference always runs BF16 MFMA. */
 CUT_EXT_EXPORT bool cut_pnet_set_precision(int mode);
+/* Network input of a block: 0 = the 16^3 template block and the 16^3 source
+ * block at the same position; 1 = the 32^3 neighbourhoods [b - 8, b + 24) of
+ * both (the +-8 search region), each averaged 2x2x2 to 16^3. */
+CUT_EXT_EXPORT bool cut_pnet_set_input_mode(int mode);
 /* Training-path forward: x [B][2][16^3] -> y [B][3]. */
 CUT_EXT_EXPORT bool cut_pnet_forward(const float* d_params, const float* d_x, int B, float* d_y);
 /* loss = (1/B) sum_b 0.5 |y_b - t_b|^2 (d_loss: one f32, may be NULL) and its

@@ -108,14 +108,49 @@ def gathered(torch_dev, pair, n, seed=0, offsets=False, flips=False):
     return desc, out
 
 
-def test_gather_normalises_blocks(torch_dev, pair):
+def pooled_region(vol, x0, y0, z0):
+    """2x2x2 means of the 32^3 region [x0 - 8, x0 + 24) per axis, coordinates clamped."""
+    n = vol.shape
+    zi = np.clip(np.arange(z0 - 8, z0 + 24), 0, n[0] - 1)
+    yi = np.clip(np.arange(y0 - 8, y0 + 24), 0, n[1] - 1)
+    xi = np.clip(np.arange(x0 - 8, x0 + 24), 0, n[2] - 1)
+    r = vol[np.ix_(zi, yi, xi)].astype(np.float64)
+    return r.reshape(16, 2, 16, 2, 16, 2).mean(axis=(1, 3, 5))
+
+
+def normalise(b, flips):
+    if flips & 1:
+        b = b[:, :, ::-1]
+    if flips & 2:
+        b = b[:, ::-1, :]
+    if flips & 4:
+        b = b[::-1, :, :]
+    b = b - b.mean()
+    sd = np.sqrt((b * b).mean())
+    return b / sd if sd > 0 else b
+
+
+@pytest.fixture(params=[0, 1])
+def input_mode(request):
+    assert api.pnet_set_input_mode(request.param)
+    yield request.param
+    assert api.pnet_set_input_mode(1)
+
+
+def test_gather_normalises_blocks(torch_dev, pair, input_mode):
     tpl, src = pair[0], pair[1]
     desc, out = gathered(torch_dev, pair, 24, seed=1, offsets=True, flips=True)
     got = out.cpu().numpy()
     for i, (_, b, ox, oy, oz, fl) in enumerate(desc):
         x0, y0, z0 = origins([b])[0]
-        assert np.abs(got[i, 0] - norm_block(tpl, x0, y0, z0, fl)).max() < 2e-4
-        assert np.abs(got[i, 1] - norm_block(src, x0 + ox, y0 + oy, z0 + oz, fl)).max() < 2e-4
+        if input_mode == 0:
+            want_t = norm_block(tpl, x0, y0, z0, fl)
+            want_s = norm_block(src, x0 + ox, y0 + oy, z0 + oz, fl)
+        else:
+            want_t = normalise(pooled_region(tpl, x0, y0, z0), fl)
+            want_s = normalise(pooled_region(src, x0 + ox, y0 + oy, z0 + oz), fl)
+        assert np.abs(got[i, 0] - want_t).max() < 2e-4
+        assert np.abs(got[i, 1] - want_s).max() < 2e-4
 
 
 @pytest.fixture(params=["fp32", "bf16"])
@@ -157,12 +192,12 @@ def test_gradients_match_oracle(torch_dev, orc, pair, precision):
     want_loss, want = orc.pnet_loss_grad(p.astype(np.float64), x.cpu().numpy(), t.astype(np.float64))
     ftol, gtol = TOL[precision]
     assert abs(loss.item() - want_loss) < ftol * 3 * want_loss
-    got = g.cpu().numpy()
+    got = g.cpu().numpy().astype(np.float64)
     errs = {name: rel_l2(got[sl], want[sl]) for name, sl in blocks_of(p)}
     print(precision, {k: round(v, 6) for k, v in errs.items()})
     assert max(errs.values()) < gtol, errs
     cos = float(got @ want / (np.linalg.norm(got) * np.linalg.norm(want)))
-    assert cos > (1 - 1e-8 if precision == "fp32" else 0.995)
+    assert cos > (1 - 1e-6 if precision == "fp32" else 0.995)
     # deterministic: the same call gives the same bits
     g2 = torch.zeros_like(g)
     assert api.pnet_loss_grad(_t(p, torch_dev), x, _t(t, torch_dev), g2)
@@ -170,7 +205,7 @@ def test_gradients_match_oracle(torch_dev, orc, pair, precision):
     assert torch.equal(g, g2)
 
 
-def test_fused_inference_matches_training_path(torch_dev, orc, pair):
+def test_fused_inference_matches_training_path(torch_dev, orc, pair, input_mode):
     import torch
     n = api.pnet_param_count()
     p = init_params(6, n)
