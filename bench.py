@@ -175,6 +175,24 @@ def main():
     ms = e0.elapsed_time(e1) / args.steps
     log(f"bench: {ms:.3f} ms/volume (wall {1000 * wall / args.steps:.3f})")
 
+    # ---- precision variant: IQ staged as FP16 / BF16 between Hilbert and DAS,
+    # coherence factor applied (FP32 accumulation throughout)
+    precision = {}
+    for name, fmt in (("f32", 0), ("fp16", 1), ("bf16", 2)):
+        assert api.set_das_options(cubic=(args.interp == "cubic"), cf_hercules=True, iq_format=fmt)
+        assert api.beamform_device(d_raw, d_vol, bp)
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(max(3, args.steps // 4)):
+            assert api.beamform_device(d_raw, d_vol, bp)
+        b.record(stream)
+        torch.cuda.synchronize()
+        pm = a.elapsed_time(b) / max(3, args.steps // 4)
+        precision[name] = {"ms_per_volume": round(pm, 4), "volumes_per_s": round(1000.0 / pm, 3)}
+    assert api.set_das_options(cubic=(args.interp == "cubic"), cf_hercules=False, iq_format=0)
+    log(f"bench: precision variants {json.dumps(precision)}")
+
     # ---- per-stage timing on the same data (stage entry points, same stream)
     def timed(fn, iters):
         fn()
@@ -258,6 +276,7 @@ def main():
                             "hilbert_GBs": round(hil_gbs, 1), "hilbert_frac_hbm": round(hil_gbs / PEAK_HBM_GBS, 4),
                             f"das_{other_interp}_TFLOPs": round(das_tflops_other, 3)},
         "cpu_baseline": cpu,
+        "precision_cf_on": precision,
         "motion": motion,
     }
     print(json.dumps(out), flush=True)
