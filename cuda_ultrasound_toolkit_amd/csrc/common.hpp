@@ -106,7 +106,19 @@ enum IqFormat { IQ_F32 = 0, IQ_F16 = 1, IQ_BF16 = 2 };
 inline size_t iq_sample_bytes(int fmt) { return fmt == IQ_F32 ? 8 : 4; }
 bool launch_hilbert(const HilbertInput& in, int rows, int S, const float2* d_tw, const float2* d_F, void* out,
                     int out_fmt, float out_scale, hipStream_t st);
-bool hilbert_supported(int S);
+bool hilbert_supported(int S);  // 1 <= S <= 4096
+bool hilbert_pow2(int S);       // powers of two 16..4096 take the radix-16 kernel
+// Tables of the chirp-z (Bluestein) path for other lengths: M = 2^bluestein_log2(S) >= 2S - 1.
+struct BluesteinTables {
+    const float2* chirp;  // S entries
+    const float2* V;      // M entries
+    const float2* Wt;     // M entries
+    const float2* tw;     // M-point twiddles
+};
+int bluestein_log2(int S);
+bool launch_bluestein_tables(int S, float2* chirp, float2* V, float2* Wt, float2* tw, hipStream_t st);
+bool launch_hilbert_bluestein(const HilbertInput& in, int rows, int S, const BluesteinTables& bt, const float2* d_F,
+                              void* out, int ofmt, float oscale, hipStream_t st);
 
 // DAS parameters in kernel form (beamformer.cu:17-98).
 struct DasParams {

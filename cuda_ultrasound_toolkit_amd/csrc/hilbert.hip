@@ -265,6 +265,163 @@ __global__ __launch_bounds__(256) void hilbert_kernel(HilbertDev in, int rows, c
     if constexpr (RT > 1) stockham_pass<S, RT, true>(tid, p, tw, load_lds, store_out, true);
 }
 
+// All passes of one N-point transform held in LDS.  The first pass reads
+// through first_load (global memory, or LDS when first_lds), the last writes
+// through last_store; a barrier follows every pass whose output later passes
+// or transforms read.
+template <int LOGN, bool INV, typename LoadF, typename StoreF>
+__device__ __forceinline__ void fft_passes(int tid, const float2* __restrict__ tw, float2* lds, LoadF first_load,
+                                           bool first_lds, StoreF last_store, bool last_global)
+{
+    constexpr int N = 1 << LOGN;
+    constexpr int NP16 = LOGN / 4;
+    constexpr int RT = 1 << (LOGN % 4);
+    auto load_lds = [&](int i) { return lds[lds_idx(i)]; };
+    auto store_lds = [&](int i, float2 v) { lds[lds_idx(i)] = v; };
+    int p = 1;
+#pragma unroll
+    for (int q = 0; q < NP16; ++q) {
+        const bool last = (q == NP16 - 1) && RT == 1;
+        if (q == 0) {
+            if (last) stockham_pass<N, 16, INV>(tid, p, tw, first_load, last_store, first_lds);
+            else stockham_pass<N, 16, INV>(tid, p, tw, first_load, store_lds, first_lds);
+        } else {
+            if (last) stockham_pass<N, 16, INV>(tid, p, tw, load_lds, last_store, true);
+            else stockham_pass<N, 16, INV>(tid, p, tw, load_lds, store_lds, true);
+        }
+        if (!(last && last_global)) __syncthreads();
+        p *= 16;
+    }
+    if constexpr (RT > 1) {
+        if (NP16 == 0) stockham_pass<N, RT, INV>(tid, p, tw, first_load, last_store, first_lds);
+        else stockham_pass<N, RT, INV>(tid, p, tw, load_lds, last_store, true);
+        if (!last_global) __syncthreads();
+    }
+}
+
+// Hilbert transform of rows of any length S (not a power of two) by
+// Bluestein's chirp-z method: with w[n] = exp(-i pi n^2 / S),
+//   DFT_S(x)[k]  = w[k] * sum_n (x[n] w[n]) conj(w)[k - n],
+//   IDFT_S(Y)[n] = conj(w[n]) * sum_k (Y[k] conj(w[k])) w[n - k],
+// each convolution evaluated as a cyclic one of length M = 2^LOGM >= 2S - 1
+// by FFTs held in LDS (V = FFT_M of the even extension of conj(w), Wt = the
+// same of w, both divided by M).  Four M-point transforms per row; the
+// spectral weight and filter are applied between them as for the
+// power-of-two kernel.
+template <int LOGM, int KIND, bool FILTER>
+__global__ __launch_bounds__(512) void hilbert_bluestein_kernel(HilbertDev in, int rows, int S,
+                                                                const float2* __restrict__ tw,
+                                                                const float2* __restrict__ chirp,
+                                                                const float2* __restrict__ V,
+                                                                const float2* __restrict__ Wt,
+                                                                const float2* __restrict__ F, void* __restrict__ out,
+                                                                int ofmt, float oscale)
+{
+    constexpr int M = 1 << LOGM;
+    constexpr int P = M / 16;
+    constexpr int RPW = P >= 256 ? 1 : 256 / P;
+    constexpr int PADDED = M + M / 16;
+    __shared__ float2 buf[RPW * PADDED];
+
+    const int tid = threadIdx.x % P;
+    const int rsub = threadIdx.x / P;
+    const int row = blockIdx.x * RPW + rsub;
+    const bool active = row < rows;
+    float2* lds = buf + rsub * PADDED;
+    const int rrow = active ? row : 0;
+    const float scale = 1.5f / (float)S;
+    const int half = S / 2;
+
+    auto load_a = [&](int n) {
+        if (n >= S || !active) return make_float2(0.0f, 0.0f);
+        const float x = load_in<KIND>(in, rrow, S, n);
+        const float2 c = chirp[n];
+        return make_float2(x * c.x, x * c.y);
+    };
+    auto load_lds = [&](int i) { return lds[lds_idx(i)]; };
+    auto store_V = [&](int k, float2 v) { lds[lds_idx(k)] = cmul(v, V[k]); };
+    auto store_W = [&](int k, float2 v) { lds[lds_idx(k)] = cmul(v, Wt[k]); };
+    // X[k] = w[k] c[k]; Y = weight * X * F for k <= S/2, 0 above; next input Y[k] conj(w[k])
+    auto store_mid = [&](int k, float2 v) {
+        float2 r = make_float2(0.0f, 0.0f);
+        if (k <= half) {
+            const float2 c = chirp[k];
+            float2 X = cmul(v, c);
+            const float wk = k == 0 ? scale * 0.5f : scale;
+            X = make_float2(X.x * wk, X.y * wk);
+            if constexpr (FILTER) X = cmul(X, F[k]);
+            r = cmul(X, make_float2(c.x, -c.y));
+        }
+        lds[lds_idx(k)] = r;
+    };
+    const size_t obase = (size_t)rrow * S;
+    auto store_out = [&](int n, float2 v) {
+        if (!active || n >= S) return;
+        const float2 c = chirp[n];
+        const float2 y = cmul(v, make_float2(c.x, -c.y));
+        if (ofmt == IQ_F32) {
+            static_cast<float2*>(out)[obase + n] = y;
+        } else if (ofmt == IQ_F16) {
+            static_cast<__half2*>(out)[obase + n] = __floats2half2_rn(y.x * oscale, y.y * oscale);
+        } else {
+            static_cast<uint32_t*>(out)[obase + n] = bf16_bits(y.x) | (bf16_bits(y.y) << 16);
+        }
+    };
+    fft_passes<LOGM, false>(tid, tw, lds, load_a, false, store_V, false);
+    fft_passes<LOGM, true>(tid, tw, lds, load_lds, true, store_mid, false);
+    fft_passes<LOGM, false>(tid, tw, lds, load_lds, true, store_W, false);
+    fft_passes<LOGM, true>(tid, tw, lds, load_lds, true, store_out, true);
+}
+
+// Bluestein tables in FP64: chirp[n] = exp(-i pi (n^2 mod 2S) / S), n < S.
+__global__ void chirp_kernel(float2* chirp, int S)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= S) return;
+    const long long q = ((long long)n * n) % (2LL * S);
+    double sn, cs;
+    sincospi((double)q / (double)S, &sn, &cs);
+    chirp[n] = make_float2((float)cs, (float)-sn);
+}
+
+// out[k] = (1/M) sum_m u[m] exp(-2 pi i m k / M) over the even extension of
+// u[m] = exp(sgn i pi m^2 / S) (m in (-S, S), index m mod M); sgn = +1 gives
+// the spectrum of conj(chirp), -1 that of chirp.
+__global__ void chirp_spectrum_kernel(float2* out, int S, int M, int sgn)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= M) return;
+    double re = 0.0, im = 0.0;
+    for (int m = -(S - 1); m < S; ++m) {
+        const long long q = ((long long)m * m) % (2LL * S);
+        const long long mm = ((long long)m % M + M) % M;
+        const long long e = (mm * k) % M;
+        // angle / pi = sgn q / S - 2 e / M
+        double sn, cs;
+        sincospi(sgn * (double)q / (double)S - 2.0 * (double)e / (double)M, &sn, &cs);
+        re += cs;
+        im += sn;
+    }
+    out[k] = make_float2((float)(re / M), (float)(im / M));
+}
+
+int bluestein_log2(int S)
+{
+    int lg = 4;
+    while ((1 << lg) < 2 * S - 1) ++lg;
+    return lg;
+}
+
+bool launch_bluestein_tables(int S, float2* chirp, float2* V, float2* Wt, float2* tw, hipStream_t st)
+{
+    const int M = 1 << bluestein_log2(S);
+    chirp_kernel<<<(S + 255) / 256, 256, 0, st>>>(chirp, S);
+    chirp_spectrum_kernel<<<(M + 127) / 128, 128, 0, st>>>(V, S, M, +1);
+    chirp_spectrum_kernel<<<(M + 127) / 128, 128, 0, st>>>(Wt, S, M, -1);
+    CUT_HIP_CHECK(hipGetLastError());
+    return launch_twiddles(tw, M, st);
+}
+
 // Twiddle table tw[m] = exp(-2 pi i m / S), generated in FP64.
 __global__ void twiddle_kernel(float2* tw, int S)
 {
@@ -306,7 +463,51 @@ bool launch_filter_spectrum(const float* d_h, int L, int S, float2* d_F, hipStre
     return true;
 }
 
-bool hilbert_supported(int S) { return S >= 16 && S <= 4096 && (S & (S - 1)) == 0; }
+bool hilbert_supported(int S) { return S >= 1 && S <= 4096; }
+bool hilbert_pow2(int S) { return S >= 16 && (S & (S - 1)) == 0; }
+
+template <int LOGM>
+static bool launch_bluestein_m(const HilbertInput& in, int rows, int S, const BluesteinTables& bt, const float2* F,
+                               void* out, int ofmt, float oscale, hipStream_t st)
+{
+    constexpr int M = 1 << LOGM;
+    constexpr int P = M / 16;
+    constexpr int RPW = P >= 256 ? 1 : 256 / P;
+    HilbertDev d{in.ptr, in.stride, in.map, in.R0, in.C};
+    dim3 grid((rows + RPW - 1) / RPW);
+    dim3 block(P * RPW);
+#define CUT_BLU(K, FL)                                                                                              \
+    hilbert_bluestein_kernel<LOGM, K, FL><<<grid, block, 0, st>>>(d, rows, S, bt.tw, bt.chirp, bt.V, bt.Wt, F, out, \
+                                                                   ofmt, oscale)
+    if (F) {
+        if (in.kind == 0) CUT_BLU(0, true); else if (in.kind == 1) CUT_BLU(1, true); else CUT_BLU(2, true);
+    } else {
+        if (in.kind == 0) CUT_BLU(0, false); else if (in.kind == 1) CUT_BLU(1, false); else CUT_BLU(2, false);
+    }
+#undef CUT_BLU
+    CUT_HIP_CHECK(hipGetLastError());
+    return true;
+}
+
+bool launch_hilbert_bluestein(const HilbertInput& in, int rows, int S, const BluesteinTables& bt, const float2* d_F,
+                              void* out, int ofmt, float oscale, hipStream_t st)
+{
+    if (rows <= 0) return true;
+    if (ofmt != IQ_F32 && ofmt != IQ_F16 && ofmt != IQ_BF16) CUT_FAIL("Hilbert: unknown output format " << ofmt);
+    switch (bluestein_log2(S)) {
+        case 4: return launch_bluestein_m<4>(in, rows, S, bt, d_F, out, ofmt, oscale, st);
+        case 5: return launch_bluestein_m<5>(in, rows, S, bt, d_F, out, ofmt, oscale, st);
+        case 6: return launch_bluestein_m<6>(in, rows, S, bt, d_F, out, ofmt, oscale, st);
+        case 7: return launch_bluestein_m<7>(in, rows, S, bt, d_F, out, ofmt, oscale, st);
+        case 8: return launch_bluestein_m<8>(in, rows, S, bt, d_F, out, ofmt, oscale, st);
+        case 9: return launch_bluestein_m<9>(in, rows, S, bt, d_F, out, ofmt, oscale, st);
+        case 10: return launch_bluestein_m<10>(in, rows, S, bt, d_F, out, ofmt, oscale, st);
+        case 11: return launch_bluestein_m<11>(in, rows, S, bt, d_F, out, ofmt, oscale, st);
+        case 12: return launch_bluestein_m<12>(in, rows, S, bt, d_F, out, ofmt, oscale, st);
+        case 13: return launch_bluestein_m<13>(in, rows, S, bt, d_F, out, ofmt, oscale, st);
+        default: CUT_FAIL("Hilbert: sample count " << S << " not supported (1..4096)");
+    }
+}
 
 template <int LOGS>
 static bool launch_hilbert_s(const HilbertInput& in, int rows, const float2* tw, const float2* F, void* out, int ofmt,

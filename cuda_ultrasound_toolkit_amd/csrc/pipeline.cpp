@@ -148,10 +148,20 @@ bool RfPipeline::init(uint raw_len, uint raw_channels, uint S, uint C, uint T, i
         CUT_HIP_CHECK(hipMemcpy(rowsum_.ptr, rs.data(), rs.size() * 4, hipMemcpyHostToDevice));
         scale_ = 1.0f / (float)T;
     }
-    // FFT plan: twiddles for the supported lengths (others fail at hilbert())
-    if (hilbert_supported((int)S)) {
+    // FFT plan: twiddles for power-of-two lengths, chirp-z tables for the
+    // others (lengths beyond 4096 fail at hilbert())
+    chirp_.release(); bV_.release(); bW_.release();
+    if (hilbert_pow2((int)S)) {
         if (!tw_.ensure((size_t)S * sizeof(float2))) return false;
         if (!launch_twiddles(tw_.as<float2>(), (int)S, stream())) return false;
+    } else if (hilbert_supported((int)S)) {
+        const size_t M = (size_t)1 << bluestein_log2((int)S);
+        if (!tw_.ensure(M * sizeof(float2)) || !chirp_.ensure((size_t)S * sizeof(float2)) ||
+            !bV_.ensure(M * sizeof(float2)) || !bW_.ensure(M * sizeof(float2)))
+            return false;
+        if (!launch_bluestein_tables((int)S, chirp_.as<float2>(), bV_.as<float2>(), bW_.as<float2>(),
+                                     tw_.as<float2>(), stream()))
+            return false;
     } else {
         tw_.release();
     }
@@ -175,6 +185,7 @@ bool RfPipeline::deinit()
     }
     init_ = false;
     h8_.release(); hf_.release(); rowsum_.release(); tw_.release();
+    chirp_.release(); bV_.release(); bW_.release();
     filter_spec_.release(); filter_taps_.release();
     filter_host_.clear();
     filter_on_ = false;
@@ -247,7 +258,9 @@ bool RfPipeline::hilbert(const HilbertInput& in, void* d_out, int fmt, float sca
     if (!init_) CUT_FAIL("Session not initialized.");
     if (!hilbert_supported((int)S_)) CUT_FAIL("Hilbert: sample count " << S_ << " is not supported by the FFT.");
     const float2* F = filter_on_ ? filter_spec_.as<float2>() : nullptr;
-    return launch_hilbert(in, (int)(C_ * T_), (int)S_, tw_.as<float2>(), F, d_out, fmt, scale, st);
+    if (hilbert_pow2((int)S_)) return launch_hilbert(in, (int)(C_ * T_), (int)S_, tw_.as<float2>(), F, d_out, fmt, scale, st);
+    const BluesteinTables bt{chirp_.as<float2>(), bV_.as<float2>(), bW_.as<float2>(), tw_.as<float2>()};
+    return launch_hilbert_bluestein(in, (int)(C_ * T_), (int)S_, bt, F, d_out, fmt, scale, st);
 }
 
 bool RfPipeline::convert_hilbert(const void* d_raw, int type, void* d_out, int fmt, float scale, hipStream_t st)

@@ -60,6 +60,7 @@ private:
     DeviceBuffer h8_, hf_, rowsum_;
     float scale_ = 1.0f;
     DeviceBuffer tw_;
+    DeviceBuffer chirp_, bV_, bW_;  // chirp-z tables (lengths that are not powers of two)
     DeviceBuffer filter_taps_, filter_spec_;
     std::vector<float> filter_host_;
     bool filter_on_ = false;

@@ -98,7 +98,10 @@ def test_convert_f32_and_i16(torch_dev, orc):
 
 
 @pytest.mark.parametrize("S,taps", [(4096, 0), (4096, 64), (1024, 0), (1024, 17), (2048, 5), (256, 0),
-                                    (64, 3), (32, 0), (16, 1)])
+                                    (64, 3), (32, 0), (16, 1),
+                                    # lengths that are not powers of two: chirp-z path
+                                    (4095, 0), (3000, 64), (2049, 5), (1234, 17), (1000, 0), (100, 7),
+                                    (17, 3), (12, 0), (3, 0), (1, 0)])
 def test_hilbert_matches_oracle(torch_dev, orc, S, taps):
     import torch
     print(f"hilbert S={S} taps={taps}", flush=True)
@@ -218,6 +221,15 @@ def test_beamform_api_rca_small(lib, orc):
     acq2.bp.decode = L.NONE
     got2 = api.beamform_volume(acq2.raw, acq2.bp)
     assert rel_l2(got2, orc.pipeline(acq2)) < 1e-4
+
+
+def test_beamform_rca_small_odd_sample_count(lib, orc):
+    """Whole path with S = 1000 (chirp-z Hilbert) against the oracle."""
+    print("rca_small S=1000", flush=True)
+    acq = configs.rca_small(seed=11, S=1000)
+    got = api.beamform_volume(acq.raw, acq.bp)
+    assert got is not None
+    assert rel_l2(got, orc.pipeline(acq)) < 1e-4
 
 
 def test_beamform_matlab_shim(lib, orc):
