@@ -88,6 +88,16 @@ class CorrImagesParameters(C.Structure):
     _fields_ = [("template_dims", C.c_uint * 2), ("source_dims", C.c_uint * 2)]
 
 
+class CutFrame(C.Structure):
+    _fields_ = [
+        ("d_raw", C.c_void_p),
+        ("raw_bytes", C.c_size_t),
+        ("d_volume", C.c_void_p),
+        ("volume_bytes", C.c_size_t),
+        ("bp", C.POINTER(CudaBeamformerParameters)),
+    ]
+
+
 class CutSynthSpec(C.Structure):
     _fields_ = [
         ("geometry", C.c_int),
@@ -158,6 +168,9 @@ def load() -> C.CDLL:
     _proto(lib, "cut_set_decode_mode", B, [C.c_int])
     _proto(lib, "cut_set_das_options", B, [C.c_int, C.c_int, C.c_int])
     _proto(lib, "cut_beamform_device", B, [V, C.c_size_t, V, C.c_size_t, P(CudaBeamformerParameters)])
+    _proto(lib, "cut_beamform_batch", B, [P(CutFrame), C.c_int, C.c_int, P(C.c_double)])
+    _proto(lib, "cut_set_autotune", B, [C.c_int])
+    _proto(lib, "cut_batch_launch_shape", B, [P(C.c_int), P(C.c_int)])
     _proto(lib, "cut_rf_init", B, [P(u), P(u), C.c_int])
     _proto(lib, "cut_rf_set_channel_mapping", B, [P(C.c_int16), u])
     _proto(lib, "cut_rf_set_match_filter", B, [P(C.c_float), u])
@@ -197,6 +210,7 @@ def exported_symbols() -> list[str]:
         "cut_beamform_device", "cut_rf_init", "cut_rf_set_channel_mapping", "cut_rf_set_match_filter",
         "cut_rf_convert_decode", "cut_rf_hilbert", "cut_das_device", "cut_das_active_taps", "cut_gl_register_device_buffers",
         "cut_ncc3d_search", "cut_envelope", "cut_rf_hilbert_format", "cut_das_device_format",
+        "cut_beamform_batch", "cut_set_autotune", "cut_batch_launch_shape",
         "cut_pnet_param_count", "cut_pnet_set_precision", "cut_pnet_set_input_mode", "cut_pnet_gather", "cut_pnet_forward", "cut_pnet_loss_grad", "cut_pnet_adam",
         "cut_pnet_infer", "cut_motion3d_learned",
         "cut_synth_rf",

@@ -244,3 +244,33 @@ def motion3d_learned(d_params, d_tpl, d_src, stride, d_disp, d_peak, radius=2, t
                                             d_peak.data_ptr(), _ptr(d_path), _ptr(d_pred),
                                             C.byref(n) if count_fallbacks else None))
     return ok, (n.value if count_fallbacks else None)
+
+
+# ---- batches of frames ------------------------------------------------------
+def beamform_batch(frames, mode: int = 0, latencies: bool = False):
+    """frames: [(d_raw, d_volume, bp)] device tensors and parameters.  mode 0:
+    persistent DAS launch inside a replayed HIP graph; 1: frame by frame.
+    Returns (ok, per-frame completion times in ms from the batch start or None)."""
+    n = len(frames)
+    arr = (L.CutFrame * n)()
+    keep = []
+    for i, (d_raw, d_vol, bp) in enumerate(frames):
+        arr[i].d_raw = d_raw.data_ptr()
+        arr[i].raw_bytes = d_raw.numel() * d_raw.element_size()
+        arr[i].d_volume = d_vol.data_ptr()
+        arr[i].volume_bytes = d_vol.numel() * d_vol.element_size()
+        arr[i].bp = C.pointer(bp)
+        keep.append(bp)
+    ms = (C.c_double * n)() if latencies else None
+    ok = bool(L.load().cut_beamform_batch(arr, n, mode, ms))
+    return ok, (list(ms) if latencies else None)
+
+
+def set_autotune(on: bool) -> bool:
+    return bool(L.load().cut_set_autotune(int(bool(on))))
+
+
+def batch_launch_shape() -> tuple[int, int]:
+    g, o = C.c_int(), C.c_int()
+    L.load().cut_batch_launch_shape(C.byref(g), C.byref(o))
+    return g.value, o.value

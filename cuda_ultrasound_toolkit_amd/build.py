@@ -28,8 +28,13 @@ COMMON = [
 ]
 
 
-def _compile(src: Path) -> Path:
+def _compile(src: Path, incremental: bool = False) -> Path:
     obj = OBJ / (src.name + ".o")
+    if incremental and obj.exists():
+        newest = max([src.stat().st_mtime] + [h.stat().st_mtime for h in CSRC.glob("*.hpp")] +
+                     [h.stat().st_mtime for h in (HERE.parent / "include").glob("*.h*")])
+        if obj.stat().st_mtime > newest:
+            return obj
     cmd = [HIPCC, *COMMON, "-c", str(src), "-o", str(obj)]
     if src.suffix == ".cpp":
         cmd[1:1] = ["-x", "hip"]
@@ -42,12 +47,14 @@ def _compile(src: Path) -> Path:
     return obj
 
 
-def build(verbose: bool = True) -> Path:
+def build(verbose: bool = True, incremental: bool = False) -> Path:
+    """Compile every source (incremental: only those newer than their object
+    or than any header) and link."""
     OBJ.mkdir(exist_ok=True)
     sources = sorted(list(CSRC.glob("*.hip")) + list(CSRC.glob("*.cpp")))
     jobs = min(len(sources), max(1, min(8, os.cpu_count() or 1)))
     with cf.ThreadPoolExecutor(jobs) as ex:
-        objs = list(ex.map(_compile, sources))
+        objs = list(ex.map(lambda p: _compile(p, incremental), sources))
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", str(tmp), *map(str, objs), "-lpthread"]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -60,4 +67,4 @@ def build(verbose: bool = True) -> Path:
 
 
 if __name__ == "__main__":
-    build()
+    build(incremental="--incremental" in sys.argv)

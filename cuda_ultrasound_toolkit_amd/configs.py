@@ -127,8 +127,8 @@ def rca_3d(seed: int = 2, T: int = 128, S: int = 4096, n: int = 128, n_scatterer
            filter_taps: int = 64, fn_rx: float = 1.5, encoding: int = L.HADAMARD, generate: bool = True,
            shift=(0.0, 0.0, 0.0), decorrelation: float = 0.0, amp_seed: int = 0,
            z_range=(5e-3, 60e-3), half_width: float = 19e-3, box_half: float = 19e-3,
-           noise: float = 0.03, focus_z: float = math.inf) -> Acquisition:
-    C = T
+           noise: float = 0.03, focus_z: float = math.inf, C: int | None = None) -> Acquisition:
+    C = T if C is None else C
     pitch = 0.3e-3
     spec = _spec(1, T, C, S, n_scatterers, pitch, (-box_half, -box_half, z_range[0]),
                  (box_half, box_half, z_range[1]), noise, seed, encoding, shift, decorrelation, amp_seed)
@@ -173,6 +173,34 @@ def rca_3d(seed: int = 2, T: int = 128, S: int = 4096, n: int = 128, n_scatterer
         raw, cmap = synth(spec, C)
     bp.channel_mapping[:C] = cmap.tolist()
     return Acquisition(bp, raw, cmap, spec)
+
+
+def uneven_frames(seed: int = 4, n: int = 64, small: bool = False, generate: bool = True) -> list:
+    """Frames of uneven size for batching: transmits (Hadamard order) in
+    {16, 32, 64, 128}, 128 receive channels, samples U{1024..4096}, 50-500
+    scatterers, output cube side U{64..160} voxels, the rca_3d chain otherwise.
+    The imaged depth follows the sample count (deepest point 90 % of the
+    round-trip range).  small: a reduced set for tests (transmits {16, 32},
+    32 channels, samples U{600..1100}, sides U{12..40}, 20-60 scatterers)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        if small:
+            T = int(rng.choice([16, 32]))
+            Cc, S = 32, int(rng.integers(600, 1101))
+            nsc, side = int(rng.integers(20, 61)), int(rng.integers(12, 41))
+            taps = 16
+        else:
+            T = int(rng.choice([16, 32, 64, 128]))
+            Cc, S = 128, int(rng.integers(1024, 4097))
+            nsc, side = int(rng.integers(50, 501)), int(rng.integers(64, 161))
+            taps = 64
+        half = 0.5 * Cc * 0.3e-3
+        zmax = min(60e-3, 0.9 * S * 1540.0 / (2 * 20e6))
+        zmin = min(5e-3, 0.25 * zmax)
+        out.append(rca_3d(seed=seed * 1000 + i, T=T, C=Cc, S=S, n=side, n_scatterers=nsc, filter_taps=taps,
+                          z_range=(zmin, zmax), half_width=half, box_half=half, generate=generate))
+    return out
 
 
 def rca_small(seed: int = 3, **kw) -> Acquisition:

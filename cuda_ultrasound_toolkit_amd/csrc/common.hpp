@@ -153,4 +153,22 @@ size_t das_workspace_bytes(const DasParams& p);
 bool launch_das(const void* iq, int iq_fmt, float iq_scale, const DasParams& p, const DasOptions& o,
                 float2* vol, void* workspace, size_t workspace_bytes, hipStream_t st);
 
+// Batched HERCULES DAS: frames of any sizes in one launch.  Items are
+// (frame, tile) pairs, tile = wz * nxt * nyt + wy * nxt + wx with wz counted
+// from the deepest tile row.  d_next: the item counter; d_done[f]: tiles of
+// frame f finished; d_finish[f]: wall-clock time at which frame f completed.
+// All frames share the template variant (focus type, apodization, IQ format).
+struct BatchFrameDev {
+    DasParams p;
+    const void* iq;
+    float2* vol;
+    float iq_inv;
+    int tiles, nxt, nyt, nzt;
+};
+void das_tile_grid(const DasParams& p, int& nxt, int& nyt, int& nzt);
+bool launch_batch_start(int* d_counters, int n, long long* d_start, hipStream_t st);
+bool launch_das_batch(const BatchFrameDev* d_frames, const int2* d_items, int n_items, int grid, int* d_next,
+                      int* d_done, long long* d_finish, const DasOptions& o, bool focus, bool sinapo, int fmt,
+                      hipStream_t st);
+
 }  // namespace cut

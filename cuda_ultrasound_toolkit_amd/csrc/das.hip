@@ -157,23 +157,17 @@ struct HercGrid {
     int nxt, nyt, nzt, splits, gt_per_split;
 };
 
+// One 16 x 16 x 8 voxel tile at (bx, by, bz) over the (group, transmit)
+// range [gt_lo, gt_hi).  Writes the final voxels (coherence factor applied
+// when CF), or, when part is not null, the partial sums (acc, incoherent) to
+// part[voxel].  All threads of the workgroup must call it (it synchronises).
 template <bool CUBIC, bool FOCUS, bool CF, bool SINAPO, int IQK>
-__global__ __launch_bounds__(256) void hercules_kernel(const void* __restrict__ iq, float iq_inv, const DasParams p,
-                                                       const HercGrid hg, float2* __restrict__ vol,
-                                                       float4* __restrict__ part)
+__device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float iq_inv, const DasParams& p, int bx,
+                                              int by, int bz, int gt_lo, int gt_hi, float2* __restrict__ vol,
+                                              float4* __restrict__ part, float2* win, int* wlo_s, int* wmode_s)
 {
-    __shared__ float2 win[HCB * HW];
-    __shared__ int wlo_s[HCB];
-    __shared__ int wmode_s[HCB];
-
     const int tid = threadIdx.x;
     const int lx = tid % HT_X, ly = tid / HT_X;
-    const int per_z = hg.nxt * hg.nyt * hg.splits;
-    const int wz = blockIdx.x / per_z;
-    const int wr = blockIdx.x - wz * per_z;
-    const int split = wr % hg.splits;
-    const int wxy = wr / hg.splits;
-    const int bx = (wxy % hg.nxt) * HT_X, by = (wxy / hg.nxt) * HT_Y, bz = (hg.nzt - 1 - wz) * HT_Z;
     const int ix = bx + lx, iy = by + ly;
     const bool col_valid = ix < p.Nx && iy < p.Ny;
 
@@ -241,8 +235,6 @@ __global__ __launch_bounds__(256) void hercules_kernel(const void* __restrict__ 
     const float lo_clamp = 1.0f, hi_clamp = (float)p.S - 2.0f;
     const size_t rowlen = (size_t)p.S;
 
-    const int gt_lo = split * hg.gt_per_split;
-    const int gt_hi = min(G * p.T, gt_lo + hg.gt_per_split);
     for (int gt = gt_lo; gt < gt_hi; ++gt) {
         {
             const int g = gt / p.T, t = gt - g * p.T;
@@ -364,8 +356,8 @@ __global__ __launch_bounds__(256) void hercules_kernel(const void* __restrict__ 
         if (iz >= p.Nz) break;
         const size_t vi = ((size_t)iz * p.Ny + iy) * p.Nx + ix;
         float2 a = acc[j];
-        if (hg.splits > 1) {
-            part[(size_t)split * p.Nx * p.Ny * p.Nz + vi] = make_float4(a.x, a.y, inc[j], 0.0f);
+        if (part) {
+            part[vi] = make_float4(a.x, a.y, inc[j], 0.0f);
             continue;
         }
         if constexpr (CF) {
@@ -376,6 +368,81 @@ __global__ __launch_bounds__(256) void hercules_kernel(const void* __restrict__ 
         }
         vol[vi] = a;
     }
+}
+
+
+template <bool CUBIC, bool FOCUS, bool CF, bool SINAPO, int IQK>
+__global__ __launch_bounds__(256) void hercules_kernel(const void* __restrict__ iq, float iq_inv, const DasParams p,
+                                                       const HercGrid hg, float2* __restrict__ vol,
+                                                       float4* __restrict__ part)
+{
+    __shared__ float2 win[HCB * HW];
+    __shared__ int wlo_s[HCB];
+    __shared__ int wmode_s[HCB];
+    const int per_z = hg.nxt * hg.nyt * hg.splits;
+    const int wz = blockIdx.x / per_z;
+    const int wr = blockIdx.x - wz * per_z;
+    const int split = wr % hg.splits;
+    const int wxy = wr / hg.splits;
+    const int bx = (wxy % hg.nxt) * HT_X, by = (wxy / hg.nxt) * HT_Y, bz = (hg.nzt - 1 - wz) * HT_Z;
+    const int G = p.readi_groups > 1 ? p.readi_groups : 1;
+    const int gt_lo = split * hg.gt_per_split;
+    const int gt_hi = min(G * p.T, gt_lo + hg.gt_per_split);
+    float4* pp = hg.splits > 1 ? part + (size_t)split * p.Nx * p.Ny * p.Nz : nullptr;
+    hercules_tile<CUBIC, FOCUS, CF, SINAPO, IQK>(iq, iq_inv, p, bx, by, bz, gt_lo, gt_hi, vol, pp, win, wlo_s,
+                                                 wmode_s);
+}
+
+// A batch of frames of any sizes in one launch.  Work items (frame, tile) are
+// listed in `items` (ordered by the host); workgroups take them in order from
+// a device counter (persistent form: gridDim.x < n_items) or one each
+// (gridDim.x == n_items).  Each item runs the whole transmit loop of its tile
+// (no split).  When a frame's last tile is done, its completion time
+// (wall clock) is recorded.
+template <bool CUBIC, bool FOCUS, bool CF, bool SINAPO, int IQK>
+__global__ __launch_bounds__(256) void hercules_batch_kernel(const BatchFrameDev* __restrict__ frames,
+                                                             const int2* __restrict__ items, int n_items,
+                                                             int* __restrict__ next, int* __restrict__ done,
+                                                             long long* __restrict__ finish)
+{
+    __shared__ float2 win[HCB * HW];
+    __shared__ int wlo_s[HCB];
+    __shared__ int wmode_s[HCB];
+    __shared__ int item_s;
+    const bool queue = gridDim.x < (unsigned)n_items;
+    int it = blockIdx.x;
+    while (true) {
+        if (queue) {
+            __syncthreads();
+            if (threadIdx.x == 0) item_s = atomicAdd(next, 1);
+            __syncthreads();
+            it = item_s;
+        }
+        if (it >= n_items) break;
+        const int2 w = items[__builtin_amdgcn_readfirstlane(it)];
+        const int f = __builtin_amdgcn_readfirstlane(w.x), tile = __builtin_amdgcn_readfirstlane(w.y);
+        const BatchFrameDev& fr = frames[f];
+        const DasParams p = fr.p;
+        const int per_z = fr.nxt * fr.nyt;
+        const int wz = tile / per_z, wxy = tile - wz * per_z;
+        const int bx = (wxy % fr.nxt) * HT_X, by = (wxy / fr.nxt) * HT_Y, bz = (fr.nzt - 1 - wz) * HT_Z;
+        const int G = p.readi_groups > 1 ? p.readi_groups : 1;
+        hercules_tile<CUBIC, FOCUS, CF, SINAPO, IQK>(fr.iq, fr.iq_inv, p, bx, by, bz, 0, G * p.T, fr.vol, nullptr,
+                                                     win, wlo_s, wmode_s);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            __threadfence();
+            if (atomicAdd(done + f, 1) == fr.tiles - 1) finish[f] = wall_clock64();
+        }
+        if (!queue) break;
+    }
+}
+
+// Zero the batch counters and record the start time.
+__global__ void batch_start_kernel(int* counters, int n, long long* start)
+{
+    for (int i = threadIdx.x; i < n; i += blockDim.x) counters[i] = 0;
+    if (threadIdx.x == 0) *start = wall_clock64();
 }
 
 // Sum of the per-split partials in split order (deterministic), then the
@@ -510,6 +577,79 @@ bool launch_herc_fo(bool fo, bool cf, bool sa, int fmt, dim3 grid, hipStream_t s
               : launch_herc_cf<CU, false>(cf, sa, fmt, grid, st, iq, inv, p, hg, vol, part);
 }
 }  // namespace
+
+namespace {
+template <bool CU, bool FO, bool CFV, bool SA>
+bool launch_batch_fmt(int fmt, int grid, hipStream_t st, const BatchFrameDev* fr, const int2* items, int n, int* next,
+                      int* done, long long* finish)
+{
+    if (fmt == IQ_F32) {
+        hercules_batch_kernel<CU, FO, CFV, SA, IQ_F32><<<grid, 256, 0, st>>>(fr, items, n, next, done, finish);
+        return true;
+    }
+    if constexpr (SA) {
+        CUT_FAIL("DAS: staged IQ formats support Hann apodization only");
+    } else {
+        if (fmt == IQ_F16)
+            hercules_batch_kernel<CU, FO, CFV, SA, IQ_F16><<<grid, 256, 0, st>>>(fr, items, n, next, done, finish);
+        else
+            hercules_batch_kernel<CU, FO, CFV, SA, IQ_BF16><<<grid, 256, 0, st>>>(fr, items, n, next, done, finish);
+        return true;
+    }
+}
+template <bool CU, bool FO, bool CFV>
+bool launch_batch_sa(bool sa, int fmt, int grid, hipStream_t st, const BatchFrameDev* fr, const int2* items, int n,
+                     int* next, int* done, long long* finish)
+{
+    return sa ? launch_batch_fmt<CU, FO, CFV, true>(fmt, grid, st, fr, items, n, next, done, finish)
+              : launch_batch_fmt<CU, FO, CFV, false>(fmt, grid, st, fr, items, n, next, done, finish);
+}
+template <bool CU, bool FO>
+bool launch_batch_cf(bool cf, bool sa, int fmt, int grid, hipStream_t st, const BatchFrameDev* fr, const int2* items,
+                     int n, int* next, int* done, long long* finish)
+{
+    return cf ? launch_batch_sa<CU, FO, true>(sa, fmt, grid, st, fr, items, n, next, done, finish)
+              : launch_batch_sa<CU, FO, false>(sa, fmt, grid, st, fr, items, n, next, done, finish);
+}
+template <bool CU>
+bool launch_batch_fo(bool fo, bool cf, bool sa, int fmt, int grid, hipStream_t st, const BatchFrameDev* fr,
+                     const int2* items, int n, int* next, int* done, long long* finish)
+{
+    return fo ? launch_batch_cf<CU, true>(cf, sa, fmt, grid, st, fr, items, n, next, done, finish)
+              : launch_batch_cf<CU, false>(cf, sa, fmt, grid, st, fr, items, n, next, done, finish);
+}
+}  // namespace
+
+void das_tile_grid(const DasParams& p, int& nxt, int& nyt, int& nzt)
+{
+    nxt = (p.Nx + HT_X - 1) / HT_X;
+    nyt = (p.Ny + HT_Y - 1) / HT_Y;
+    nzt = (p.Nz + HT_Z - 1) / HT_Z;
+}
+
+bool launch_batch_start(int* d_counters, int n, long long* d_start, hipStream_t st)
+{
+    batch_start_kernel<<<1, 256, 0, st>>>(d_counters, n, d_start);
+    CUT_HIP_CHECK(hipGetLastError());
+    return true;
+}
+
+bool launch_das_batch(const BatchFrameDev* d_frames, const int2* d_items, int n_items, int grid, int* d_next,
+                      int* d_done, long long* d_finish, const DasOptions& o, bool focus, bool sinapo, int fmt,
+                      hipStream_t st)
+{
+    if (n_items <= 0) return true;
+    if (fmt != IQ_F32 && fmt != IQ_F16 && fmt != IQ_BF16) CUT_FAIL("DAS: unknown IQ format " << fmt);
+    grid = std::max(1, std::min(grid, n_items));
+    const bool cf = o.cf_hercules != 0;
+    const bool ok = o.cubic ? launch_batch_fo<true>(focus, cf, sinapo, fmt, grid, st, d_frames, d_items, n_items,
+                                                    d_next, d_done, d_finish)
+                            : launch_batch_fo<false>(focus, cf, sinapo, fmt, grid, st, d_frames, d_items, n_items,
+                                                     d_next, d_done, d_finish);
+    if (!ok) return false;
+    CUT_HIP_CHECK(hipGetLastError());
+    return true;
+}
 
 bool launch_das(const void* iq, int iq_fmt, float iq_scale, const DasParams& p, const DasOptions& o, float2* vol,
                 void* workspace, size_t workspace_bytes, hipStream_t st)

@@ -464,8 +464,8 @@ bool validate_beamform(const CudaBeamformerParameters& bp, size_t raw_bytes, siz
     return true;
 }
 
-bool beamform_device(RfPipeline& rf, BeamformBuffers& bufs, const void* d_raw, size_t raw_bytes, float2* d_vol,
-                     size_t vol_bytes, const CudaBeamformerParameters& bp, hipStream_t st)
+bool prepare_frame(RfPipeline& rf, BeamformBuffers& bufs, const CudaBeamformerParameters& bp, size_t raw_bytes,
+                   size_t vol_bytes, FramePlan& plan)
 {
     if (!validate_beamform(bp, raw_bytes, vol_bytes)) return false;
     const uint R0 = bp.rf_raw_dim[0], RC = bp.rf_raw_dim[1];
@@ -474,31 +474,46 @@ bool beamform_device(RfPipeline& rf, BeamformBuffers& bufs, const void* d_raw, s
     if (!rf.init(R0, RC, S, C, T, bp.decode)) CUT_FAIL("Failed to initialize RF processor.");
     if (!rf.set_channel_mapping(bp.channel_mapping, C)) CUT_FAIL("Failed to set channel mapping.");
     if (!rf.set_match_filter(bp.rf_filter, bp.filter_length)) CUT_FAIL("Failed to set match filter.");
+    if (!das_params_from_bp(bp, plan.p)) return false;
 
-    DasParams p;
-    if (!das_params_from_bp(bp, p)) return false;
-
-    const int fmt = options().iq_format;
-    if (fmt != IQ_F32 && p.sequence != HERCULES) CUT_FAIL("Staged IQ formats are supported for HERCULES only.");
-    if (fmt == IQ_F16 && bp.data_type != TYPE_I16) CUT_FAIL("FP16-staged IQ needs int16 input data.");
-    const float scale = fmt == IQ_F16 ? rf.fp16_scale() : 1.0f;
+    plan.fmt = options().iq_format;
+    if (plan.fmt != IQ_F32 && plan.p.sequence != HERCULES) CUT_FAIL("Staged IQ formats are supported for HERCULES only.");
+    if (plan.fmt == IQ_F16 && bp.data_type != TYPE_I16) CUT_FAIL("FP16-staged IQ needs int16 input data.");
+    plan.scale = plan.fmt == IQ_F16 ? rf.fp16_scale() : 1.0f;
+    plan.opts.cubic = options().das_cubic;
+    plan.opts.cf_hercules = options().das_cf_hercules;
     const size_t count = (size_t)S * C * T;
-    if (!bufs.iq.ensure(count * iq_sample_bytes(fmt))) return false;
+    if (!bufs.iq.ensure(count * iq_sample_bytes(plan.fmt))) return false;
+    if (bp.decode != NONE && !bufs.dec.ensure(count * sizeof(float))) return false;
+    return true;
+}
+
+bool frame_to_iq(RfPipeline& rf, BeamformBuffers& bufs, const void* d_raw, const CudaBeamformerParameters& bp,
+                 const FramePlan& plan, hipStream_t st)
+{
+    const uint C = bp.dec_data_dim[1];
     void* iq = bufs.iq.ptr;
     if (bp.decode != NONE) {
-        if (!bufs.dec.ensure(count * sizeof(float))) return false;
         if (!rf.convert_decode(d_raw, bp.data_type, bufs.dec.as<float>(), 1, true, st)) CUT_FAIL("Failed to decode RF data.");
         HilbertInput in{bufs.dec.ptr, 0, 1, nullptr, 0, (int)C};
-        if (!rf.hilbert(in, iq, fmt, scale, st)) CUT_FAIL("Failed to apply Hilbert transform.");
+        if (!rf.hilbert(in, iq, plan.fmt, plan.scale, st)) CUT_FAIL("Failed to apply Hilbert transform.");
     } else {
-        if (!rf.convert_hilbert(d_raw, bp.data_type, iq, fmt, scale, st)) CUT_FAIL("Failed to apply Hilbert transform.");
+        if (!rf.convert_hilbert(d_raw, bp.data_type, iq, plan.fmt, plan.scale, st))
+            CUT_FAIL("Failed to apply Hilbert transform.");
     }
-    DasOptions o;
-    o.cubic = options().das_cubic;
-    o.cf_hercules = options().das_cf_hercules;
-    const size_t ws = das_workspace_bytes(p);
+    return true;
+}
+
+bool beamform_device(RfPipeline& rf, BeamformBuffers& bufs, const void* d_raw, size_t raw_bytes, float2* d_vol,
+                     size_t vol_bytes, const CudaBeamformerParameters& bp, hipStream_t st)
+{
+    FramePlan plan;
+    if (!prepare_frame(rf, bufs, bp, raw_bytes, vol_bytes, plan)) return false;
+    if (!frame_to_iq(rf, bufs, d_raw, bp, plan, st)) return false;
+    const size_t ws = das_workspace_bytes(plan.p);
     if (ws && !bufs.das_ws.ensure(std::max(ws, bufs.das_ws.bytes))) return false;
-    if (!launch_das(iq, fmt, scale, p, o, d_vol, bufs.das_ws.ptr, bufs.das_ws.bytes, st)) CUT_FAIL("Beamforming failed.");
+    if (!launch_das(bufs.iq.ptr, plan.fmt, plan.scale, plan.p, plan.opts, d_vol, bufs.das_ws.ptr, bufs.das_ws.bytes, st))
+        CUT_FAIL("Beamforming failed.");
     return true;
 }
 

@@ -74,6 +74,19 @@ struct BeamformBuffers {
 bool validate_beamform(const CudaBeamformerParameters& bp, size_t raw_bytes, size_t vol_bytes);
 bool beamform_device(RfPipeline& rf, BeamformBuffers& bufs, const void* d_raw, size_t raw_bytes, float2* d_vol,
                      size_t vol_bytes, const CudaBeamformerParameters& bp, hipStream_t st);
+// The same in two halves: host-side set-up of one frame (validation, RF state,
+// DAS constants, buffers; may synchronise), then the decode + Hilbert launches
+// (launches only, so they can be captured into a graph).
+struct FramePlan {
+    DasParams p;
+    DasOptions opts;
+    int fmt = IQ_F32;
+    float scale = 1.0f;
+};
+bool prepare_frame(RfPipeline& rf, BeamformBuffers& bufs, const CudaBeamformerParameters& bp, size_t raw_bytes,
+                   size_t vol_bytes, FramePlan& plan);
+bool frame_to_iq(RfPipeline& rf, BeamformBuffers& bufs, const void* d_raw, const CudaBeamformerParameters& bp,
+                 const FramePlan& plan, hipStream_t st);
 
 unsigned long long das_active_taps(const DasParams& p);
 

@@ -50,6 +50,30 @@ CUT_EXT_EXPORT bool cut_set_das_options(int cubic, int cf_hercules, int iq_forma
 CUT_EXT_EXPORT bool cut_beamform_device(const void* d_raw, size_t raw_bytes, void* d_volume, size_t volume_bytes,
                                         const CudaBeamformerParameters* bp);
 
+/* ---- batches of frames of any sizes ------------------------------------- */
+typedef struct CutFrame {
+    const void* d_raw;       /* raw frame in device memory, as cuda_toolkit::beamform's input */
+    size_t raw_bytes;
+    void* d_volume;          /* complex64 [Nz][Ny][Nx] */
+    size_t volume_bytes;
+    const CudaBeamformerParameters* bp;
+} CutFrame;
+/* Beamform n frames (each with its own transmit count, sample count, grid...)
+ * and wait for them.  mode 0: every frame's decode and Hilbert, then one
+ * persistent HERCULES DAS launch over the work of the whole batch, captured
+ * into a HIP graph that is replayed while pointers, sizes and options stay the
+ * same (frames must be HERCULES and share focus type and apodization); mode
+ * 1: the single-frame path frame after frame.  Each frame keeps its own RF
+ * state and buffers across calls.  frame_ms (may be NULL): per frame, the time
+ * from the start of the batch on the GPU to that frame's completion. */
+CUT_EXT_EXPORT bool cut_beamform_batch(const CutFrame* frames, int n, int mode, double* frame_ms);
+/* Autotuning of the batch DAS launch (grid size and work order), on by
+ * default: timed once per batch shape, the fastest candidate is kept. */
+CUT_EXT_EXPORT bool cut_set_autotune(int on);
+/* Launch shape the last graph-mode batch used: resident workgroups per CU
+ * (0 = one workgroup per tile) and work order (0 heaviest first, 1 frame by frame). */
+CUT_EXT_EXPORT bool cut_batch_launch_shape(int* grid_per_cu, int* order);
+
 /* ---- stages on device pointers (async), one RF state of their own ------- */
 /* RfProcessor::init (cuda_toolkit/src/rf_processing/rf_processor.cpp:21-71):
  * raw_dims = {samples*transmits + pad, raw channels}; dec_dims = {S, C, T}. */
