@@ -121,6 +121,53 @@ def motion_leg(dev, stream, seed=20261203, repeats=5):
     return out
 
 
+def batch_leg(dev, n, seed=4, repeats=3):
+    """n frames of uneven size (transmits {16..128}, 128 channels, samples
+    U{1024..4096}, cube side U{64..160}): one graph-captured batch with a
+    persistent DAS launch vs the same frames one after another."""
+    import numpy as np
+    import torch
+
+    from cuda_ultrasound_toolkit_amd import api, configs
+
+    log(f"bench: synthesizing {n} uneven frames")
+    t0 = time.perf_counter()
+    acqs = configs.uneven_frames(seed=seed, n=n)
+    log(f"bench: synthesis {time.perf_counter() - t0:.1f} s")
+    assert api.set_das_options(cubic=False)
+    raws = [torch.from_numpy(a.raw).to(dev) for a in acqs]
+    vols = [torch.zeros(int(np.prod(a.bp.output_points[:3])) * 2, dtype=torch.float32, device=dev) for a in acqs]
+    frames = [(r, v, a.bp) for r, v, a in zip(raws, vols, acqs)]
+    res = {}
+    for name, mode in (("graph_persistent", 0), ("sequential", 1)):
+        ok, _ = api.beamform_batch(frames, mode=mode)  # set-up, tuning, capture
+        assert ok
+        lat, wall = [], []
+        for _ in range(repeats):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            ok, ms = api.beamform_batch(frames, mode=mode, latencies=True)
+            wall.append(time.perf_counter() - t)
+            assert ok
+            lat.append(ms)
+        gpu_ms = [max(m) for m in lat]
+        best = int(np.argmin(gpu_ms))
+        m = np.array(lat[best])
+        res[name] = {"batch_gpu_ms": round(gpu_ms[best], 3), "volumes_per_s": round(n / (gpu_ms[best] * 1e-3), 2),
+                     "wall_ms": round(1000 * min(wall), 3),
+                     "latency_p50_ms": round(float(np.percentile(m, 50)), 3),
+                     "latency_p99_ms": round(float(np.percentile(m, 99)), 3)}
+    g, o = api.batch_launch_shape()
+    res["launch_shape"] = {"workgroups_per_cu": g, "order": ["heaviest first", "frame by frame"][o]}
+    res["speedup"] = round(res["sequential"]["batch_gpu_ms"] / res["graph_persistent"]["batch_gpu_ms"], 3)
+    res["workload"] = (f"{n} frames, seed {seed}: transmits {{16,32,64,128}}, 128 channels, samples U{{1024..4096}}, "
+                       "50-500 scatterers, cube side U{64..160}, linear interpolation")
+    res["total_voxels"] = int(sum(int(np.prod(a.bp.output_points[:3])) for a in acqs))
+    assert api.set_das_options(cubic=True)
+    log(f"bench: batch {json.dumps(res)}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -130,6 +177,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--stage-iters", type=int, default=10)
     ap.add_argument("--no-motion", action="store_true", help="skip the 3-D motion leg (learned vs full search)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-frame uneven batch leg")
+    ap.add_argument("--batch-frames", type=int, default=64)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one device-to-device copy of the 512 MiB IQ buffer (counter calibration)")
     args = ap.parse_args()
@@ -227,6 +276,7 @@ def main():
         torch.cuda.synchronize()
         log(f"bench: calibration copy of {d_iq.numel() * 4} bytes")
     motion = None if args.no_motion else motion_leg(dev, stream)
+    batch = None if args.no_batch else batch_leg(dev, args.batch_frames)
     api.set_stream(None)
 
     taps = C.c_ulonglong()
@@ -278,6 +328,7 @@ def main():
         "cpu_baseline": cpu,
         "precision_cf_on": precision,
         "motion": motion,
+        "batch": batch,
     }
     print(json.dumps(out), flush=True)
 
