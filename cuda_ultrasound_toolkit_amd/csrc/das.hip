@@ -186,6 +186,7 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
     const float fz = p.focal_z;
     // per-voxel constants
     float z2[HT_Z], kz[HT_Z], base[HT_Z];
+    double base_d[HT_Z];  // the same in FP64, anchored per channel block below
     bool zvalid[HT_Z];
 #pragma unroll
     for (int j = 0; j < HT_Z; ++j) {
@@ -202,6 +203,7 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
             txd = z;  // (0, y sin 0, z cos 0): steering angle zero
         }
         base[j] = (txd + fz) * p.spm + (float)p.delay;
+        base_d[j] = ((double)txd + (double)fz) * (double)p.spm + (double)p.delay;
     }
     // transmit path bounds over the tile
     float txd_lo, txd_hi;
@@ -295,54 +297,74 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
                     }
                 }
                 __syncthreads();
-                for (int cc = 0; cc < nC; ++cc) {
-                    const int c = cb + cc;
-                    const float ex = fmaf((float)c, p.pitch_x, p.xdc_min_x);
+                // per channel: lateral distance, apodization inputs, and whether any
+                // lane of the wave can be inside the aperture (uniform skip)
+                auto chan = [&](int cc, float& rho2, float& rho, float& apo_c) -> bool {
+                    const float ex = fmaf((float)(cb + cc), p.pitch_x, p.xdc_min_x);
                     const float dx = ex - vx;
-                    const float rho2 = fmaf(dx, dx, dy2);
-                    float rho = 0.0f, apo_c = 0.0f;
+                    rho2 = fmaf(dx, dx, dy2);
+                    rho = 0.0f;
+                    apo_c = 0.0f;
                     if constexpr (SINAPO) {
                         apo_c = sin_apo_to(ex, p.xdc_max_x, p.to_power) * apo_y;
-                        if (!(apo_c > 0.1f)) continue;  // uniform: depends on the element only
+                        return apo_c > 0.1f;  // uniform: depends on the element only
                     } else {
                         rho = fsqrt(rho2);
-                        // the deepest voxel of the column has the widest aperture:
-                        // skip the channel when it is outside for every lane
-                        if (!__any(rho * kz[HT_Z - 1] < HANN_A_THR * 1.001f || p.fn_rx <= 0.0f)) continue;
+                        // the deepest voxel of the column has the widest aperture
+                        return __any(rho * kz[HT_Z - 1] < HANN_A_THR * 1.001f || p.fn_rx <= 0.0f);
                     }
-                    const int wlo = wlo_s[cc];
-                    if (wmode_s[cc] == 0) {
-                        const float2* wrow = win + cc * HW;
-                        // indices relative to the window: base - wlo, clamp bounds shifted alike
-                        const float fwlo = (float)wlo;
-                        const float lo_w = lo_clamp - fwlo, hi_w = hi_clamp - fwlo;
+                };
+                // Sample indices of this block of channels are taken relative to
+                // the smallest window start of the block (an integer, so fractions
+                // are unchanged and indices stay small and non-negative); the
+                // per-channel window offset then moves into the uniform LDS base.
+                int org = 0x7fffffff;
+                for (int cc = 0; cc < nC; ++cc)
+                    if (wmode_s[cc] == 0) org = min(org, wlo_s[cc]);
+                const float forg = (float)org;
+                const float lo_r = lo_clamp - forg, hi_r = hi_clamp - forg;
+                float base_r[HT_Z];
 #pragma unroll
-                        for (int j = 0; j < HT_Z; ++j) {
-                            const float w = tap_weight<SINAPO>(rho, kz[j], apo_c);
-                            // the window covers every voxel of the tile with a 2-sample margin,
-                            // so the local index needs no further clamp
-                            const float x = __builtin_amdgcn_fmed3f(fmaf(fsqrt(rho2 + z2[j]), p.spm, base[j] - fwlo), lo_w, hi_w);
-                            const float tt = __builtin_amdgcn_fractf(x);
-                            const int li = (int)x;
-                            float2 v;
-                            if constexpr (CUBIC) v = cubic2(wrow[li - 1], wrow[li], wrow[li + 1], wrow[li + 2], tt);
-                            else v = lerp2(wrow[li], wrow[li + 1], tt);
-                            acc[j].x = fmaf(w, v.x, acc[j].x);
-                            acc[j].y = fmaf(w, v.y, acc[j].y);
-                            if constexpr (CF) inc[j] = (w != 0.0f) ? fmaf(v.x, v.x, fmaf(v.y, v.y, inc[j])) : inc[j];
-                        }
-                    } else {
-                        const size_t grow = toff + (size_t)c * rowlen;
+                for (int j = 0; j < HT_Z; ++j) base_r[j] = (float)(base_d[j] - (double)org);
+                // channels whose window sits in LDS; a single path keeps the
+                // accumulators in place across iterations
+                for (int cc = 0; cc < nC; ++cc) {
+                    if (wmode_s[cc] != 0) continue;
+                    float rho2, rho, apo_c;
+                    if (!chan(cc, rho2, rho, apo_c)) continue;
+                    // window row of channel cc, indexed by block-relative sample index
+                    const float2* wrow = win + __builtin_amdgcn_readfirstlane(cc * HW - (wlo_s[cc] - org));
 #pragma unroll
-                        for (int j = 0; j < HT_Z; ++j) {
-                            const float w = tap_weight<SINAPO>(rho, kz[j], apo_c);
-                            if (w == 0.0f) continue;
-                            const float x = tap_index(rho2 + z2[j], p.spm, base[j], hi_clamp);
-                            const float2 v = interp_global<CUBIC, IQK>(iq, grow, x, p.S, iq_inv);
-                            acc[j].x = fmaf(w, v.x, acc[j].x);
-                            acc[j].y = fmaf(w, v.y, acc[j].y);
-                            if constexpr (CF) inc[j] = fmaf(v.x, v.x, fmaf(v.y, v.y, inc[j]));
-                        }
+                    for (int j = 0; j < HT_Z; ++j) {
+                        const float w = tap_weight<SINAPO>(rho, kz[j], apo_c);
+                        // the window covers every voxel of the tile with a 2-sample margin,
+                        // so the index needs no clamp beyond the reference's [1, S-2]
+                        const float x = __builtin_amdgcn_fmed3f(fmaf(fsqrt(rho2 + z2[j]), p.spm, base_r[j]), lo_r, hi_r);
+                        const float tt = __builtin_amdgcn_fractf(x);
+                        const int li = (int)x;
+                        float2 v;
+                        if constexpr (CUBIC) v = cubic2(wrow[li - 1], wrow[li], wrow[li + 1], wrow[li + 2], tt);
+                        else v = lerp2(wrow[li], wrow[li + 1], tt);
+                        acc[j].x = fmaf(w, v.x, acc[j].x);
+                        acc[j].y = fmaf(w, v.y, acc[j].y);
+                        if constexpr (CF) inc[j] = (w != 0.0f) ? fmaf(v.x, v.x, fmaf(v.y, v.y, inc[j])) : inc[j];
+                    }
+                }
+                // channels whose window did not fit: direct reads
+                for (int cc = 0; cc < nC; ++cc) {
+                    if (wmode_s[cc] == 0) continue;
+                    float rho2, rho, apo_c;
+                    if (!chan(cc, rho2, rho, apo_c)) continue;
+                    const size_t grow = toff + (size_t)(cb + cc) * rowlen;
+#pragma unroll
+                    for (int j = 0; j < HT_Z; ++j) {
+                        const float w = tap_weight<SINAPO>(rho, kz[j], apo_c);
+                        if (w == 0.0f) continue;
+                        const float x = tap_index(rho2 + z2[j], p.spm, base[j], hi_clamp);
+                        const float2 v = interp_global<CUBIC, IQK>(iq, grow, x, p.S, iq_inv);
+                        acc[j].x = fmaf(w, v.x, acc[j].x);
+                        acc[j].y = fmaf(w, v.y, acc[j].y);
+                        if constexpr (CF) inc[j] = fmaf(v.x, v.x, fmaf(v.y, v.y, inc[j]));
                     }
                 }
             }
