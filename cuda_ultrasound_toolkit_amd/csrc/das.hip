@@ -167,7 +167,10 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
                                               float4* __restrict__ part, float2* win, int* wlo_s, int* wmode_s)
 {
     const int tid = threadIdx.x;
-    const int lx = tid % HT_X, ly = tid / HT_X;
+    // each wave owns an 8 x 8 patch of columns: a compact footprint keeps the
+    // wave's aperture test coherent, so more channels are skipped whole
+    const int wv = tid >> 6, ln = tid & 63;
+    const int lx = (wv & 1) * 8 + (ln & 7), ly = (wv >> 1) * 8 + (ln >> 3);
     const int ix = bx + lx, iy = by + ly;
     const bool col_valid = ix < p.Nx && iy < p.Ny;
 
