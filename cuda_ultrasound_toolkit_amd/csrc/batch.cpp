@@ -145,6 +145,7 @@ extern "C" bool cut_set_autotune(int on)
 {
     std::lock_guard<std::mutex> lock(cut::api_mutex());
     batch_state().autotune = on ? 1 : 0;
+    cut::set_das_autotune(on != 0);
     return true;
 }
 

@@ -147,6 +147,9 @@ struct DasOptions {
 // HERCULES splits the transmit loop over workgroups and needs a workspace for
 // the partial sums (das_workspace_bytes; 0 when no split is used).
 int das_splits(long long tiles, int gt_total);
+// Autotuning of the transmit split (on by default; the workspace then covers
+// the largest candidate).
+void set_das_autotune(bool on);
 size_t das_workspace_bytes(const DasParams& p);
 // iq_fmt: an IqFormat; the stored samples are value * iq_scale.  Staged
 // (F16/BF16) IQ is supported by the HERCULES kernel with Hann apodization.

@@ -20,8 +20,11 @@
 
 #include <hip/hip_fp16.h>
 
-#include <cmath>
 #include <algorithm>
+#include <array>
+#include <cmath>
+#include <map>
+#include <mutex>
 #include <type_traits>
 
 namespace cut {
@@ -542,12 +545,36 @@ __global__ __launch_bounds__(256) void forces_kernel(const void* __restrict__ iq
     vol[v] = make_float2(acc.x * cf, acc.y * cf);
 }
 
+// Transmit-split tuning of the single-frame HERCULES launch: the heuristic
+// das_splits, or, with autotuning on, the fastest of 1, 2, 4 and 8 splits,
+// measured the first time a shape is seen (outside stream capture).  Every
+// candidate computes the whole volume; only the order in which the per-split
+// partial sums are added differs.
+namespace {
+std::mutex g_tune_mutex;
+std::map<std::array<long long, 12>, int> g_tuned_splits;
+bool g_das_autotune = true;
+constexpr int TUNE_MAX_SPLITS = 8;
+
+int max_splits(int gt_total) { return std::max(1, std::min(TUNE_MAX_SPLITS, gt_total / 8)); }
+}  // namespace
+
+void set_das_autotune(bool on)
+{
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    g_das_autotune = on;
+}
+
 size_t das_workspace_bytes(const DasParams& p)
 {
     if (p.sequence != HERCULES) return 0;
     const long long tiles = (long long)((p.Nx + HT_X - 1) / HT_X) * ((p.Ny + HT_Y - 1) / HT_Y) * ((p.Nz + HT_Z - 1) / HT_Z);
     const int G = p.readi_groups > 1 ? p.readi_groups : 1;
-    const int splits = das_splits(tiles, G * p.T);
+    int splits = das_splits(tiles, G * p.T);
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mutex);
+        if (g_das_autotune) splits = std::max(splits, max_splits(G * p.T));
+    }
     return splits > 1 ? (size_t)splits * p.Nx * p.Ny * p.Nz * sizeof(float4) : 0;
 }
 
@@ -691,27 +718,67 @@ bool launch_das(const void* iq, int iq_fmt, float iq_scale, const DasParams& p, 
         hg.nzt = (p.Nz + HT_Z - 1) / HT_Z;
         const long long tiles = (long long)hg.nxt * hg.nyt * hg.nzt;
         const int G = p.readi_groups > 1 ? p.readi_groups : 1;
-        hg.splits = das_splits(tiles, G * p.T);
-        hg.gt_per_split = (G * p.T + hg.splits - 1) / hg.splits;
         const size_t nvox = (size_t)p.Nx * p.Ny * p.Nz;
-        float4* part = nullptr;
-        if (hg.splits > 1) {
-            if (!workspace || workspace_bytes < (size_t)hg.splits * nvox * sizeof(float4))
-                CUT_FAIL("DAS: workspace of " << (size_t)hg.splits * nvox * sizeof(float4) << " bytes needed");
-            part = (float4*)workspace;
-        }
-        if (tiles * hg.splits > 0x7fffffffLL) CUT_FAIL("DAS: grid too large");
-        dim3 grid((unsigned)(tiles * hg.splits));
         const bool focus = p.focus_type == 4;
         const bool sinapo = p.apo_type == RX_TO_SIN;
         const bool cf = o.cf_hercules != 0;
-        const bool ok = o.cubic ? launch_herc_fo<true>(focus, cf, sinapo, iq_fmt, grid, st, iq, inv, p, hg, vol, part)
-                                : launch_herc_fo<false>(focus, cf, sinapo, iq_fmt, grid, st, iq, inv, p, hg, vol, part);
-        if (!ok) return false;
-        CUT_HIP_CHECK(hipGetLastError());
-        if (hg.splits > 1) {
-            hercules_reduce<<<(unsigned)((nvox + 255) / 256), 256, 0, st>>>(part, hg.splits, nvox, cf ? 1 : 0, p.cw, vol);
+        auto run = [&](int splits) -> bool {
+            hg.splits = splits;
+            hg.gt_per_split = (G * p.T + splits - 1) / splits;
+            float4* part = nullptr;
+            if (splits > 1) {
+                if (!workspace || workspace_bytes < (size_t)splits * nvox * sizeof(float4))
+                    CUT_FAIL("DAS: workspace of " << (size_t)splits * nvox * sizeof(float4) << " bytes needed");
+                part = (float4*)workspace;
+            }
+            if (tiles * splits > 0x7fffffffLL) CUT_FAIL("DAS: grid too large");
+            dim3 grid((unsigned)(tiles * splits));
+            const bool ok = o.cubic ? launch_herc_fo<true>(focus, cf, sinapo, iq_fmt, grid, st, iq, inv, p, hg, vol, part)
+                                    : launch_herc_fo<false>(focus, cf, sinapo, iq_fmt, grid, st, iq, inv, p, hg, vol, part);
+            if (!ok) return false;
+            CUT_HIP_CHECK(hipGetLastError());
+            if (splits > 1)
+                hercules_reduce<<<(unsigned)((nvox + 255) / 256), 256, 0, st>>>(part, splits, nvox, cf ? 1 : 0, p.cw, vol);
+            return true;
+        };
+        int splits = das_splits(tiles, G * p.T);
+        const std::array<long long, 12> key{p.Nx, p.Ny, p.Nz, p.T, p.C, p.S, G, iq_fmt, o.cubic, cf, focus, sinapo};
+        bool tune = false;
+        {
+            std::lock_guard<std::mutex> lock(g_tune_mutex);
+            auto it = g_tuned_splits.find(key);
+            if (it != g_tuned_splits.end()) splits = it->second;
+            else tune = g_das_autotune;
         }
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (tune && hipStreamIsCapturing(st, &cs) == hipSuccess && cs == hipStreamCaptureStatusNone &&
+            (size_t)max_splits(G * p.T) * nvox * sizeof(float4) <= workspace_bytes) {
+            hipEvent_t a, b;
+            CUT_HIP_CHECK(hipEventCreate(&a));
+            CUT_HIP_CHECK(hipEventCreate(&b));
+            float best = 1e30f;
+            int best_s = splits;
+            for (int cand = 1; cand <= max_splits(G * p.T); cand *= 2) {
+                float ms = 0.0f;
+                const bool ok = hipEventRecord(a, st) == hipSuccess && run(cand) && hipEventRecord(b, st) == hipSuccess &&
+                                hipEventSynchronize(b) == hipSuccess && hipEventElapsedTime(&ms, a, b) == hipSuccess;
+                if (!ok) {
+                    (void)hipEventDestroy(a);
+                    (void)hipEventDestroy(b);
+                    CUT_FAIL("DAS: split tuning failed");
+                }
+                if (ms < best) {
+                    best = ms;
+                    best_s = cand;
+                }
+            }
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+            std::lock_guard<std::mutex> lock(g_tune_mutex);
+            g_tuned_splits[key] = best_s;
+            return true;  // the last candidate left a complete volume
+        }
+        if (!run(splits)) return false;
     } else if (p.sequence == FORCES) {
         if (p.readi_groups > 1 && p.encoding != HADAMARD) CUT_FAIL("FORCES only supports Hadamard encoding at this time.");
         if (iq_fmt != IQ_F32) CUT_FAIL("DAS: staged IQ formats are supported for HERCULES only");

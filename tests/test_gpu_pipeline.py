@@ -184,6 +184,26 @@ def test_das_hercules_sin_apodization(torch_dev, orc):
     _das_case(acq, orc, torch_dev, True, 1e-4)
 
 
+@pytest.mark.parametrize("G,gid,enc", [(2, 1, L.HADAMARD), (4, 2, L.HADAMARD), (4, 3, L.WALSH)])
+def test_das_hercules_readi_groups(torch_dev, orc, G, gid, enc):
+    """READI: G groups of the transmit aperture, the group's sign row from the
+    order-G matrix (templated_beamformer.cuh:43-63)."""
+    print(f"das hercules READI G={G} id={gid} enc={enc}", flush=True)
+    acq = configs.rca_small(seed=23)
+    acq.bp.readi_group_count = G
+    acq.bp.readi_group_id = gid
+    acq.bp.decode = enc
+    _das_case(acq, orc, torch_dev, cubic=True, tol=1e-4)
+
+
+def test_das_forces_readi_groups(torch_dev, orc):
+    print("das forces READI G=2", flush=True)
+    acq = configs.linear_2d(seed=4)
+    acq.bp.readi_group_count = 2
+    acq.bp.readi_group_id = 1
+    _das_case(acq, orc, torch_dev, cubic=False, tol=1e-4)
+
+
 @pytest.mark.parametrize("cubic", [True, False])
 def test_das_forces_matches_oracle(torch_dev, orc, cubic):
     print(f"das forces cubic={cubic}", flush=True)
