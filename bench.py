@@ -121,6 +121,47 @@ def motion_leg(dev, stream, seed=20261203, repeats=5):
     return out
 
 
+def legacy_leg(dev, stream, repeats=3):
+    """UFORCES (2-D, 32-element linear array) and RCA_TPW (3-D, 32 tilted
+    plane waves x 128 columns -> 128^3) DAS kernels on device-resident IQ."""
+    import ctypes as C
+    import torch
+
+    from cuda_ultrasound_toolkit_amd import _lib as L, api, configs
+    lib = L.load()
+    out = {}
+    cases = {
+        "uforces_2d": (configs.uforces_2d(seed=6, T=32, S=1024, generate=False),
+                       "UFORCES: flash + 31 sparse transmits x 32 ch x 1024 samples -> 128 x 256, cubic"),
+        "tpw_3d": (configs.tpw_3d(seed=2, T=32, C=128, S=4096, n=128, generate=False, z_range=(5e-3, 60e-3),
+                                  half_width=19e-3, box_half=19e-3),
+                   "RCA_TPW: 32 angles x 128 ch x 4096 samples -> 128^3, cubic, F# 1.5 Hann"),
+    }
+    for name, (acq, desc) in cases.items():
+        bp = acq.bp
+        S, Cc, T = bp.dec_data_dim[0], bp.dec_data_dim[1], bp.dec_data_dim[2]
+        n = bp.output_points
+        g = torch.Generator(device=dev).manual_seed(11)
+        d_iq = torch.randn(T * Cc * S * 2, device=dev, generator=g)
+        d_vol = torch.zeros(n[0] * n[1] * n[2] * 2, dtype=torch.float32, device=dev)
+        assert api.das_device(d_iq, d_vol, bp)
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(repeats):
+            assert api.das_device(d_iq, d_vol, bp)
+        b.record(stream)
+        torch.cuda.synchronize()
+        ms = a.elapsed_time(b) / repeats
+        taps = C.c_ulonglong()
+        assert lib.cut_das_active_taps(C.byref(bp), C.byref(taps))
+        out[name] = {"workload": desc, "ms": round(ms, 4), "active_taps": taps.value,
+                     "gtaps_per_s": round(taps.value / (ms * 1e-3) / 1e9, 2)}
+        del d_iq, d_vol
+    log(f"bench: legacy sequences {json.dumps(out)}")
+    return out
+
+
 def batch_leg(dev, n, seed=4, repeats=3):
     """n frames of uneven size (transmits {16..128}, 128 channels, samples
     U{1024..4096}, cube side U{64..160}): one graph-captured batch with a
@@ -179,6 +220,7 @@ def main():
     ap.add_argument("--no-motion", action="store_true", help="skip the 3-D motion leg (learned vs full search)")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-frame uneven batch leg")
     ap.add_argument("--batch-frames", type=int, default=64)
+    ap.add_argument("--no-legacy", action="store_true", help="skip the UFORCES / RCA_TPW kernel timings")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one device-to-device copy of the 512 MiB IQ buffer (counter calibration)")
     args = ap.parse_args()
@@ -276,6 +318,7 @@ def main():
         torch.cuda.synchronize()
         log(f"bench: calibration copy of {d_iq.numel() * 4} bytes")
     motion = None if args.no_motion else motion_leg(dev, stream)
+    legacy = None if args.no_legacy else legacy_leg(dev, stream)
     batch = None if args.no_batch else batch_leg(dev, args.batch_frames)
     api.set_stream(None)
 
@@ -329,6 +372,7 @@ def main():
         "precision_cf_on": precision,
         "motion": motion,
         "batch": batch,
+        "legacy_sequences": legacy,
     }
     print(json.dumps(out), flush=True)
 

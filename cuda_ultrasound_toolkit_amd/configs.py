@@ -211,3 +211,40 @@ def rca_small(seed: int = 3, **kw) -> Acquisition:
     args.update(kw)
     acq = rca_3d(seed=seed, **args)
     return acq
+
+
+def uforces_2d(seed: int = 6, T: int = 16, S: int = 1024, nx: int = 128, nz: int = 256, step: int = 2,
+               fn_rx: float = 1.0, generate: bool = True) -> Acquisition:
+    """UFORCES on the linear_2d array: the first of T transmits is a flash
+    that the beamformer discards, transmit t + 1 is fired from element
+    ``step * t`` (sparse_elements), diverging from the array (transmit and
+    receive on the same rows: focus at depth 0).  Hann receive apodization at
+    F# ``fn_rx``.  The RF is the linear_2d synthesizer's (Hadamard-T)."""
+    acq = linear_2d(seed=seed, T=T, S=S, nx=nx, nz=nz, generate=generate)
+    bp = acq.bp
+    bp.das_shader_id = L.UFORCES
+    bp.tx_orientation = L.ORIENT_ROWS
+    bp.rx_orientation = L.ORIENT_ROWS
+    bp.foci[0], bp.foci[1], bp.foci[2] = 0.0, 0.0, 0.0
+    bp.fn_rx = fn_rx
+    for t in range(T):
+        bp.sparse_elements[t] = step * t
+    return acq
+
+
+def tpw_3d(seed: int = 7, T: int = 16, C: int = 32, S: int = 1024, n: int = 24, max_angle_deg: float = 10.0,
+           generate: bool = True, **kw) -> Acquisition:
+    """RCA_TPW: T tilted plane waves from the rows (steering angles evenly
+    spaced in [-max, max] degrees, passed in radians in foci[3t+2]), received
+    on C columns, Hann receive apodization.  The RF is the rca_3d
+    synthesizer's at the same sizes."""
+    args = dict(T=T, C=C, S=S, n=n, n_scatterers=40, filter_taps=16, z_range=(3e-3, 14e-3),
+                half_width=0.15e-3 * C, box_half=0.15e-3 * C)
+    args.update(kw)
+    acq = rca_3d(seed=seed, generate=generate, **args)
+    bp = acq.bp
+    bp.das_shader_id = L.RCA_TPW
+    ang = np.deg2rad(np.linspace(-max_angle_deg, max_angle_deg, T))
+    for t in range(T):
+        bp.foci[3 * t + 0], bp.foci[3 * t + 1], bp.foci[3 * t + 2] = 0.0, 0.0, float(ang[t])
+    return acq

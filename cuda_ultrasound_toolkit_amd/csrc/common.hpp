@@ -137,6 +137,11 @@ struct DasParams {
     int readi_groups, encoding;
     unsigned long long readi_bits;
     int sequence;
+    // per-transmit geometry of UFORCES and RCA_TPW (beamformer_kernels.cu):
+    // UFORCES: tx_tab[t].x = sparse_elements[t] - sparse_elements[0];
+    // RCA_TPW: tx_tab[t].x = steering angle foci[3t+2] [rad]
+    double fc_fs;  // centre / sampling frequency (RCA_TPW phase rotation)
+    float2 tx_tab[MAX_ACQ_COUNT];
 };
 bool das_params_from_bp(const CudaBeamformerParameters& bp, DasParams& p);
 

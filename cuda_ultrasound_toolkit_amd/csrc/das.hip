@@ -545,6 +545,202 @@ __global__ __launch_bounds__(256) void forces_kernel(const void* __restrict__ iq
     vol[v] = make_float2(acc.x * cf, acc.y * cf);
 }
 
+// ---------------------------------------------------------------------------
+// UFORCES and RCA_TPW (beamformer_kernels.cu:94-172, :489-546): one thread
+// per voxel, receive element e at x_min + p/2 + e p on the x axis (x_min is
+// already the first element's centre, so the reference's extra half pitch is
+// kept), coherence factor over the apodized taps.
+// ---------------------------------------------------------------------------
+
+// Weight of the first used UFORCES transmit (beamformer_kernels.cu:148-151).
+constexpr float UF_FIRST_SCALE = 0.088388347648318f;
+
+__device__ __forceinline__ float2 apply_cf(float2 acc, float inc, float cw)
+{
+    float cf = powf((acc.x * acc.x + acc.y * acc.y) / inc, cw);
+    if (isnan(cf)) cf = 0.0f;
+    return make_float2(acc.x * cf, acc.y * cf);
+}
+
+// UFORCES: data row block t + 1 (block 0 is a discarded flash) was fired from
+// element sparse_elements[t].  The transmit vector is calc_tx_distance(vox,
+// focal, type) (beamformer_utils.cuh:8-32) with focal = (x_min + p/2, vox.y,
+// 0), shifted by tx_tab[t].x pitches in x; path |rx| + |tx| sign(vox.z),
+// index clamped to [0, S-2], Catmull-Rom over the flat buffer, Hann receive
+// weight without a threshold (taps of weight cos^2(pi/2) are skipped).
+//
+// Precision as in the RCA_TPW kernel below: per voxel and transmit the
+// channel-independent part (z + |tx| sign) spm + delay is FP64 and split
+// into an integer and a fraction; per channel only (|rx| - z) spm =
+// rx_x^2 / (|rx| + z) spm is FP32.
+template <bool CUBIC>
+__global__ __launch_bounds__(256) void uforces_kernel(const float2* __restrict__ iq, const DasParams p,
+                                                      float2* __restrict__ vol)
+{
+    const long long nvox = (long long)p.Nx * p.Ny * p.Nz;
+    const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nvox) return;
+    const int ix = (int)(v % p.Nx), iy = (int)((v / p.Nx) % p.Ny), iz = (int)(v / ((long long)p.Nx * p.Ny));
+    const double vxd = (double)p.min_x + ix * (double)p.res_x;
+    const double vyd = (double)p.min_y + iy * (double)p.res_y;
+    const double vzd = (double)p.min_z + iz * (double)p.res_z;
+    const double half = 0.5 * (double)p.pitch_x;
+    const double fx = (double)p.xdc_min_x + half;
+    double tx0 = 0.0, ty = 0.0, tz = 0.0;
+    if (p.focus_type == 3) {         // XZ_FOCUS at depth 0
+        tx0 = fx - vxd;
+        tz = -vzd;
+    } else if (p.focus_type == 4) {  // YZ_FOCUS: focal.y = vox.y
+        tz = -vzd;
+    } else if (p.focus_type == 1) {  // XZ_PLANE: focal.x taken as the angle
+        tx0 = vxd * sin(fx);
+        tz = vzd * cos(fx);
+    } else if (p.focus_type == 2) {  // YZ_PLANE: focal.y taken as the angle
+        ty = vyd * sin(vyd);
+        tz = vzd * cos(vyd);
+    }
+    const double sgn = copysign(1.0, vzd), tyz2 = fma(ty, ty, tz * tz);
+    const float vz = (float)vzd, vz2 = vz * vz;
+    const float rx0 = (float)((double)p.xdc_min_x - vxd + half);
+    const int n = p.T * p.C * p.S;
+    float2 acc = make_float2(0.0f, 0.0f);
+    float inc = 0.0f;
+    for (int t = 0; t + 1 < p.T; ++t) {
+        const double txx = fma((double)p.tx_tab[t].x, (double)p.pitch_x, tx0);
+        const double X0 = (vzd + sqrt(fma(txx, txx, tyz2)) * sgn) * (double)p.spm + (double)p.delay;
+        const double I0 = fmin(fmax(floor(X0), -1048576.0), 1048576.0);
+        const float f0 = (float)(X0 - I0), lo = (float)(-I0), hi = (float)((double)p.S - 2.0 - I0);
+        const int i0 = (int)I0;
+        const float sc = t == 0 ? UF_FIRST_SCALE : 1.0f;
+        const int row0 = (t + 1) * p.C * p.S;
+        for (int e = 0; e < p.C; ++e) {
+            const float rxx = fmaf((float)e, p.pitch_x, rx0);
+            const float a = p.fn_rx * (fabsf(rxx) / vz) * 0.5f;
+            const bool in = a < 0.5f;
+            if (!__any(in)) continue;
+            const float w = in ? hann_from_a(a) * sc : 0.0f;
+            const float rr = rxx * rxx;
+            const float den = fsqrt(rr + vz2) + vz;
+            const float excess = den > 0.0f ? rr * __builtin_amdgcn_rcpf(den) * p.spm : 0.0f;
+            const float x = __builtin_amdgcn_fmed3f(f0 + excess, lo, hi);
+            const float fl = floorf(x);
+            const float tt = x - fl;
+            // sample in [0, S-2] of row (t + 1, e); the integer clamp only acts
+            // on non-finite input; j = -1 reads the previous row's last sample
+            const int b = row0 + e * p.S + clampi(i0 + (int)fl, 0, p.S - 2);
+            float2 val;
+            if constexpr (CUBIC)
+                val = cubic2(iq[b - 1], iq[b], iq[b + 1], iq[min(b + 2, n - 1)], tt);
+            else
+                val = lerp2(iq[b], iq[b + 1], tt);
+            const float qx = val.x * w, qy = val.y * w;
+            acc.x += qx;
+            acc.y += qy;
+            inc = fmaf(qx, qx, fmaf(qy, qy, inc));
+        }
+    }
+    vol[v] = apply_cf(acc, inc, p.cw);
+}
+
+// Steering table of RCA_TPW in FP64: (sin, cos) of each transmit's angle.
+__global__ void tpw_table_kernel(const DasParams p, double2* __restrict__ sc)
+{
+    for (int t = threadIdx.x; t < p.T; t += blockDim.x) {
+        double s, c;
+        sincos((double)p.tx_tab[t].x, &s, &c);
+        sc[t] = make_double2(s, c);
+    }
+}
+
+constexpr int TPW_QB = 8;  // angles per register block
+
+// RCA_TPW: tilted plane waves with steering (sin, cos) = sc[a] along y
+// (YZ_PLANE) or x (XZ_PLANE); channels of Hann weight below 0.1 skipped; index
+// clamped to [1, S-2]; Catmull-Rom read; the sample is rotated by
+// exp(+2 pi i x fc/fs) (rotate_iq, :482-487) before the weight.
+//
+// Precision: the index is about (z + lat sin + z cos) spm, up to a few
+// thousand, where an FP32 sum would round to ~1e-4 samples.  Per voxel and
+// angle the part that does not depend on the channel, X0 = (z + lat sin +
+// z cos) spm + delay, is formed in FP64 and split into an integer I0 and a
+// fraction; per channel only the receive excess over the depth,
+// (|rx| - z) spm = rx_x^2 / (|rx| + z) spm (at most a few hundred samples),
+// is FP32.  The tap's index is I0 + (fraction + excess), the clamp bounds are
+// moved by I0 (integers, exact), and the rotation's phase is frac(I0 fc/fs)
+// (FP64) + (fraction + excess) fc/fs.
+template <bool CUBIC>
+__global__ __launch_bounds__(256) void tpw_kernel(const float2* __restrict__ iq, const DasParams p,
+                                                  const double2* __restrict__ sc, float2* __restrict__ vol)
+{
+    const long long nvox = (long long)p.Nx * p.Ny * p.Nz;
+    const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nvox) return;
+    const int ix = (int)(v % p.Nx), iy = (int)((v / p.Nx) % p.Ny), iz = (int)(v / ((long long)p.Nx * p.Ny));
+    const double vxd = (double)p.min_x + ix * (double)p.res_x;
+    const double vyd = (double)p.min_y + iy * (double)p.res_y;
+    const double vzd = (double)p.min_z + iz * (double)p.res_z;
+    const double latd = p.focus_type == 2 ? vyd : vxd;
+    const double spmd = p.spm;
+    const float vz = (float)vzd, vz2 = vz * vz;
+    const float rx0 = (float)((double)p.xdc_min_x + 0.5 * (double)p.pitch_x - vxd);
+    const float fcfs = (float)p.fc_fs;
+    const int n = p.T * p.C * p.S;
+    float2 acc = make_float2(0.0f, 0.0f);
+    float inc = 0.0f;
+    for (int q0 = 0; q0 < p.T; q0 += TPW_QB) {
+        const int nq = min(TPW_QB, p.T - q0);
+        float f0[TPW_QB], ph0[TPW_QB], lo[TPW_QB], hi[TPW_QB];
+        int i0[TPW_QB], row[TPW_QB];
+#pragma unroll
+        for (int j = 0; j < TPW_QB; ++j) {
+            const int q = min(q0 + j, p.T - 1);
+            const double2 s = sc[q];
+            const double X0 = fma(latd, s.x, fma(vzd, s.y, vzd)) * spmd + (double)p.delay;
+            // bounded so that every integer below stays in range for any input
+            const double I0 = fmin(fmax(floor(X0), -1048576.0), 1048576.0);
+            const double pI = I0 * p.fc_fs;
+            f0[j] = (float)(X0 - I0);
+            ph0[j] = (float)(pI - floor(pI));
+            lo[j] = (float)(1.0 - I0);
+            hi[j] = (float)((double)p.S - 2.0 - I0);
+            i0[j] = (int)I0;
+            row[j] = q * p.C * p.S;
+        }
+        for (int c = 0; c < p.C; ++c) {
+            const float rxx = fmaf((float)c, p.pitch_x, rx0);
+            const float a = p.fn_rx * (fabsf(rxx) / vz) * 0.5f;
+            // cos^2(pi min(a, 1/2)) >= 0.1  <=>  a < acos(sqrt(0.1)) / pi
+            const bool in = a < HANN_A_THR;
+            if (!__any(in)) continue;
+            const float w = in ? hann_from_a(a) : 0.0f;
+            const float rr = rxx * rxx;
+            const float den = fsqrt(rr + vz2) + vz;
+            const float excess = den > 0.0f ? rr * __builtin_amdgcn_rcpf(den) * p.spm : 0.0f;
+            const int crow = c * p.S;
+#pragma unroll
+            for (int j = 0; j < TPW_QB; ++j) {
+                if (j >= nq) break;
+                const float x = __builtin_amdgcn_fmed3f(f0[j] + excess, lo[j], hi[j]);
+                const float fl = floorf(x);
+                const float t = x - fl;
+                // sample in [1, S-2] of row (q0 + j, c); the integer clamp only
+                // acts on non-finite input
+                const int i = row[j] + crow + clampi(i0[j] + (int)fl, 1, p.S - 2);
+                float2 val;
+                if constexpr (CUBIC) val = cubic2(iq[i - 1], iq[i], iq[i + 1], iq[min(i + 2, n - 1)], t);
+                else val = lerp2(iq[i], iq[i + 1], t);
+                const float ph = __builtin_amdgcn_fractf(fmaf(x, fcfs, ph0[j]));
+                const float cs = __builtin_amdgcn_cosf(ph), sn = __builtin_amdgcn_sinf(ph);
+                const float qx = (val.x * cs - val.y * sn) * w, qy = (val.x * sn + val.y * cs) * w;
+                acc.x += qx;
+                acc.y += qy;
+                inc = fmaf(qx, qx, fmaf(qy, qy, inc));
+            }
+        }
+    }
+    vol[v] = apply_cf(acc, inc, p.cw);
+}
+
 // Transmit-split tuning of the single-frame HERCULES launch: the heuristic
 // das_splits, or, with autotuning on, the fastest of 1, 2, 4 and 8 splits,
 // measured the first time a shape is seen (outside stream capture).  Every
@@ -567,6 +763,7 @@ void set_das_autotune(bool on)
 
 size_t das_workspace_bytes(const DasParams& p)
 {
+    if (p.sequence == RCA_TPW) return (size_t)MAX_ACQ_COUNT * sizeof(double2);
     if (p.sequence != HERCULES) return 0;
     const long long tiles = (long long)((p.Nx + HT_X - 1) / HT_X) * ((p.Ny + HT_Y - 1) / HT_Y) * ((p.Nz + HT_Z - 1) / HT_Z);
     const int G = p.readi_groups > 1 ? p.readi_groups : 1;
@@ -786,6 +983,28 @@ bool launch_das(const void* iq, int iq_fmt, float iq_scale, const DasParams& p, 
         dim3 grid((unsigned)((nvox + 255) / 256));
         if (o.cubic) forces_kernel<true><<<grid, 256, 0, st>>>(iq, p, vol);
         else forces_kernel<false><<<grid, 256, 0, st>>>(iq, p, vol);
+    } else if (p.sequence == UFORCES || p.sequence == RCA_TPW) {
+        if (iq_fmt != IQ_F32) CUT_FAIL("DAS: staged IQ formats are supported for HERCULES only");
+        if (p.T > MAX_ACQ_COUNT) CUT_FAIL("DAS: at most " << MAX_ACQ_COUNT << " transmits");
+        if ((long long)p.T * p.C * p.S > 0x7fffffffLL) CUT_FAIL("DAS: IQ buffer too large");
+        if (p.sequence == RCA_TPW && p.focus_type != 1 && p.focus_type != 2)
+            CUT_FAIL("TPW beamforming requires a plane focus.");
+        if (p.sequence == UFORCES && p.focus_type == 0) CUT_FAIL("Beamformer: UFORCES needs a transmit orientation.");
+        const long long nvox = (long long)p.Nx * p.Ny * p.Nz;
+        dim3 grid((unsigned)((nvox + 255) / 256));
+        const float2* in = static_cast<const float2*>(iq);
+        if (p.sequence == UFORCES) {
+            if (o.cubic) uforces_kernel<true><<<grid, 256, 0, st>>>(in, p, vol);
+            else uforces_kernel<false><<<grid, 256, 0, st>>>(in, p, vol);
+        } else {
+            if (!workspace || workspace_bytes < (size_t)p.T * sizeof(double2))
+                CUT_FAIL("DAS: workspace of " << (size_t)p.T * sizeof(double2) << " bytes needed");
+            double2* sc = static_cast<double2*>(workspace);
+            tpw_table_kernel<<<1, 256, 0, st>>>(p, sc);
+            if (o.cubic) tpw_kernel<true><<<grid, 256, 0, st>>>(in, p, sc, vol);
+            else tpw_kernel<false><<<grid, 256, 0, st>>>(in, p, sc, vol);
+        }
+        CUT_HIP_CHECK(hipGetLastError());
     } else {
         CUT_FAIL("Beamformer: Unsupported sequence ID " << p.sequence);
     }

@@ -332,6 +332,17 @@ bool das_params_from_bp(const CudaBeamformerParameters& bp, DasParams& p)
     p.cw = std::min(std::max(bp.coherency_weighting, 0.0f), 1.0f);
     p.apo_type = bp.apo_type;
     p.to_power = bp.to_power;
+    p.fc_fs = (double)bp.center_frequency / (double)bp.sampling_frequency;
+    if (p.sequence == UFORCES || p.sequence == RCA_TPW) {
+        if (p.T > MAX_ACQ_COUNT) CUT_FAIL("Beamformer: at most " << MAX_ACQ_COUNT << " transmits.");
+        for (int t = 0; t < p.T; ++t) {
+            if (p.sequence == UFORCES) {
+                p.tx_tab[t] = make_float2((float)(bp.sparse_elements[t] - bp.sparse_elements[0]), 0.0f);
+            } else {
+                p.tx_tab[t] = make_float2(bp.foci[3 * t + 2], 0.0f);
+            }
+        }
+    }
     p.readi_bits = 0;
     if (p.readi_groups > 1) {
         if (p.encoding == NONE) CUT_FAIL("Beamformer: READI groups need an encoding matrix.");
@@ -369,6 +380,23 @@ unsigned long long das_active_taps(const DasParams& p)
                 }
             }
         return n * p.Ny;
+    }
+    if (p.sequence == UFORCES || p.sequence == RCA_TPW) {
+        // receive element c at x_min + p/2 + c p; UFORCES evaluates every tap
+        // of nonzero Hann weight over T - 1 transmits, RCA_TPW the channels of
+        // weight >= 0.1 over all T angles
+        const bool uf = p.sequence == UFORCES;
+        unsigned long long n = 0;
+        for (int iz = 0; iz < p.Nz; ++iz)
+            for (int ix = 0; ix < p.Nx; ++ix) {
+                const double vx = (double)p.min_x + ix * (double)p.res_x, vz = (double)p.min_z + iz * (double)p.res_z;
+                for (int c = 0; c < p.C; ++c) {
+                    const double rx = (double)p.xdc_min_x - vx + 0.5 * p.pitch_x + c * (double)p.pitch_x;
+                    const double a = p.fn_rx * (std::fabs(rx) / vz) / 2.0;
+                    if (uf ? a < 0.5 : hann(std::fabs(rx), vz, p.fn_rx) >= 0.1) ++n;
+                }
+            }
+        return n * p.Ny * (unsigned long long)(uf ? std::max(0, p.T - 1) : p.T);
     }
     const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
     std::vector<unsigned long long> part(nt, 0);
@@ -458,6 +486,11 @@ bool validate_beamform(const CudaBeamformerParameters& bp, size_t raw_bytes, siz
         if (p.focus_type != 2 && p.focus_type != 4) CUT_FAIL("Invalid focus for HERCULES.");
     } else if (p.sequence == FORCES) {
         if (p.readi_groups > 1 && p.encoding != HADAMARD) CUT_FAIL("FORCES only supports Hadamard encoding at this time.");
+    } else if (p.sequence == UFORCES) {
+        if (bp.sparse_elements[0] == -1) CUT_FAIL("Beamformer: UFORCES requires sparse elements to be defined.");
+        if (p.focus_type == 0) CUT_FAIL("Beamformer: UFORCES needs a transmit orientation.");
+    } else if (p.sequence == RCA_TPW) {
+        if (p.focus_type != 1 && p.focus_type != 2) CUT_FAIL("TPW beamforming requires a plane focus.");
     } else {
         CUT_FAIL("Beamformer: Unsupported sequence ID " << p.sequence);
     }

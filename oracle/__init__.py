@@ -35,6 +35,9 @@ class DasConsts(C.Structure):
         ("readi_groups", C.c_int), ("readi_id", C.c_int), ("encoding", C.c_int),
         ("apo_type", C.c_int), ("to_power", C.c_int),
         ("sequence", C.c_int),
+        ("fs", C.c_float), ("fc", C.c_float),
+        ("tx_angle", C.c_float * 256),
+        ("tx_element", C.c_int * 256),
     ]
 
 

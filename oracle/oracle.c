@@ -322,6 +322,12 @@ int or_das_constants(const CudaBeamformerParameters* bp, or_das_consts* k)
     k->cw = cw < 0.0f ? 0.0f : (cw > 1.0f ? 1.0f : cw);
     k->apo_type = bp->apo_type;
     k->to_power = bp->to_power;
+    k->fs = bp->sampling_frequency;
+    k->fc = bp->center_frequency;
+    for (int t = 0; t < MAX_ACQ_COUNT; ++t) {
+        k->tx_angle[t] = bp->foci[3 * t + 2];
+        k->tx_element[t] = bp->sparse_elements[t];
+    }
     return 0;
 }
 
@@ -390,6 +396,132 @@ static void interp_row(const double* row, int S, double x, int linear, double* v
 
 static double clampd(double v, double lo, double hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+/* beamformer_utils.cuh:142-154 with fminf's NaN rule (a NaN ratio, from a
+ * voxel at depth 0 under the element, clamps to 1/2). */
+static double hann_fmin(double lateral, double depth, double fnum)
+{
+    double a = fmin(fnum * (lateral / depth) / 2.0, 0.5);
+    a = cos(M_PI * a);
+    return a * a;
+}
+
+/* beamformer_utils.cuh:56-97 (cubic_spline) and :127-140 (lerp_read) as the
+ * UFORCES kernel calls them: sample j of the row at `base` is element
+ * base + j of the flat [T][C][S] buffer, so j = -1 reads the previous row's
+ * last sample.  Elements past the buffer's end (reached only with zero
+ * weight, at x = S-2) read 0. */
+static void interp_flat(const double* iq, size_t n, size_t base, double x, int linear, double* vr, double* vi)
+{
+    long long i = (long long)floor(x);
+    double t = x - (double)i;
+#define Q(j, c) ((long long)base + (j) >= 0 && (size_t)((long long)base + (j)) < n ? iq[2 * (size_t)((long long)base + (j)) + (c)] : 0.0)
+    if (linear) {
+        *vr = Q(i, 0) + t * (Q(i + 1, 0) - Q(i, 0));
+        *vi = Q(i, 1) + t * (Q(i + 1, 1) - Q(i, 1));
+        return;
+    }
+    double t2 = t * t;
+    double h00 = (2.0 * t - 3.0) * t2 + 1.0;
+    double h10 = (t - 2.0) * t2 + t;
+    double h01 = (-2.0 * t + 3.0) * t2;
+    double h11 = (t - 1.0) * t2;
+    for (int c = 0; c < 2; ++c) {
+        double m0 = 0.5 * (Q(i + 1, c) - Q(i - 1, c));
+        double m1 = 0.5 * (Q(i + 2, c) - Q(i, c));
+        double v = h00 * Q(i, c) + h01 * Q(i + 1, c) + h10 * m0 + h11 * m1;
+        if (c == 0) *vr = v; else *vi = v;
+    }
+#undef Q
+}
+
+/* Taps of a UFORCES voxel with nonzero Hann weight (ratio below 1/2): the
+ * ones the GPU kernel evaluates.  The others carry cos^2(pi/2), 0 to within
+ * rounding in the reference. */
+static int fnum_ratio_inside(double lateral, double depth, double fnum)
+{
+    return fnum * (lateral / depth) / 2.0 < 0.5;
+}
+
+/* UFORCES voxel (beamformer_kernels.cu:94-172).  Transmit t + 1 (row block 0
+ * is a discarded flash) is fired from element sparse_elements[t]; the
+ * transmit vector starts as calc_tx_distance(vox, focal, type)
+ * (beamformer_utils.cuh:8-32) with focal = (x_min + p/2, vox.y, 0) and moves
+ * by (sparse_elements[t] - sparse_elements[0]) pitches in x; receive element e
+ * sits at x_min + p/2 + e p.  Path = |rx| + |tx| sign(vox.z), index clamped to
+ * [0, S-2], Catmull-Rom over the flat buffer, 1/sqrt(128) on the first used
+ * transmit, Hann receive apodization without a threshold, and the incoherent
+ * sum over the apodized values. */
+static void uforces_voxel(const double* iq, const or_das_consts* k, double vx, double vy, double vz, int interp,
+                          double* ar, double* ai, double* inc, long long* act)
+{
+    const int S = k->S, C = k->C, T = k->T;
+    const size_t n = (size_t)T * C * S;
+    const double px = k->pitch_x;
+    const double fx = (double)k->xdc_min_x + px / 2.0;
+    double tx0 = 0.0, ty = 0.0, tz = 0.0;
+    switch (k->focus_type) {
+        case 3: tx0 = fx - vx; tz = -vz; break;                       /* XZ_FOCUS, focus depth 0 */
+        case 4: tz = -vz; break;                                      /* YZ_FOCUS: focal.y = vox.y */
+        case 1: tx0 = vx * sin(fx); tz = vz * cos(fx); break;         /* XZ_PLANE: focal.x as the angle */
+        case 2: ty = vy * sin(vy); tz = vz * cos(vy); break;          /* YZ_PLANE: focal.y as the angle */
+        default: break;
+    }
+    const double sgn = copysign(1.0, vz);
+    const double rx0 = (double)k->xdc_min_x - vx + px / 2.0;
+    for (int t = 0; t + 1 < T; ++t) {
+        const double txx = tx0 + (double)(k->tx_element[t] - k->tx_element[0]) * px;
+        const double txd = sqrt(txx * txx + ty * ty + tz * tz) * sgn;
+        const double scale = t == 0 ? 1.0 / sqrt(128.0) : 1.0;
+        for (int e = 0; e < C; ++e) {
+            const double rxx = rx0 + (double)e * px;
+            const double apo = hann_fmin(fabs(rxx), vz, k->fn_rx);
+            const double idx = clampd((sqrt(rxx * rxx + vz * vz) + txd) * (double)k->spm + k->delay, 0.0, S - 2.0);
+            double pr, pi;
+            interp_flat(iq, n, ((size_t)(t + 1) * C + e) * S, idx, interp, &pr, &pi);
+            pr *= scale * apo;
+            pi *= scale * apo;
+            *ar += pr; *ai += pi;
+            *inc += pr * pr + pi * pi;
+            if (fnum_ratio_inside(fabs(rxx), vz, k->fn_rx)) ++*act;
+        }
+    }
+}
+
+/* RCA_TPW voxel (beamformer_kernels.cu:489-546): tilted plane waves with
+ * steering angle foci[3a+2] along y (YZ_PLANE) or x (XZ_PLANE); receive
+ * element c at x_min + p/2 + c p on the x axis.  Channels with Hann
+ * apodization below 0.1 are skipped; index clamped to [1, S-2]; Catmull-Rom
+ * read; the sample is rotated by exp(+2 pi i index fc / fs) (rotate_iq,
+ * :482-487) before apodization; incoherent sum over the apodized values. */
+static void tpw_voxel(const double* iq, const or_das_consts* k, double vx, double vy, double vz, int interp,
+                      double* ar, double* ai, double* inc, long long* act)
+{
+    const int S = k->S, C = k->C, T = k->T;
+    const double px = k->pitch_x;
+    const double lat = k->focus_type == 2 ? vy : vx;
+    const double rx0 = (double)k->xdc_min_x - vx + px / 2.0;
+    const double f = (double)k->fc / (double)k->fs;
+    for (int c = 0; c < C; ++c) {
+        const double rxx = rx0 + (double)c * px;
+        const double rxd = sqrt(rxx * rxx + vz * vz);
+        const double apo = hann_fmin(fabs(rxx), vz, k->fn_rx);
+        if (apo < 0.1) continue;
+        for (int a = 0; a < T; ++a) {
+            const double th = k->tx_angle[a];
+            const double txd = lat * sin(th) + vz * cos(th);
+            const double idx = clampd((rxd + txd) * (double)k->spm + k->delay, 1.0, S - 2.0);
+            double pr, pi;
+            interp_row(iq + 2 * ((size_t)a * C + c) * S, S, idx, interp, &pr, &pi);
+            const double ph = 2.0 * M_PI * idx * f;
+            const double cr = cos(ph), ci = sin(ph);
+            const double qr = (pr * cr - pi * ci) * apo, qi = (pr * ci + pi * cr) * apo;
+            *ar += qr; *ai += qi;
+            *inc += qr * qr + qi * qi;
+            ++*act;
+        }
+    }
+}
+
 /* HERCULES: templated_beamformer.cuh:102-216; FORCES: :219-324. */
 int or_das(const double* iq, const or_das_consts* k, int interp, int cf_mode, double* out, long long* active)
 {
@@ -398,9 +530,13 @@ int or_das(const double* iq, const or_das_consts* k, int interp, int cf_mode, do
     uint64_t row_bits = 0;
     if (G > 1 && or_readi_row_bits(k, &row_bits) != 0) return -1;
     const int hercules = k->sequence == HERCULES;
-    if (!hercules && k->sequence != FORCES) return -1;
+    const int legacy = k->sequence == UFORCES || k->sequence == RCA_TPW;
+    if (!hercules && !legacy && k->sequence != FORCES) return -1;
     if (hercules && k->focus_type != 2 && k->focus_type != 4) return -1;
-    if (!hercules && G > 1 && k->encoding != HADAMARD) return -1;
+    if (k->sequence == FORCES && G > 1 && k->encoding != HADAMARD) return -1;
+    if (legacy && (T > MAX_ACQ_COUNT || k->focus_type == 0)) return -1;
+    if (k->sequence == RCA_TPW && k->focus_type != 1 && k->focus_type != 2) return -1;
+    if (k->sequence == UFORCES && k->tx_element[0] == -1) return -1;
     const long long nvox = (long long)k->Nx * k->Ny * k->Nz;
     long long act_total = 0;
 #pragma omp parallel for schedule(dynamic, 64) reduction(+ : act_total)
@@ -413,7 +549,11 @@ int or_das(const double* iq, const or_das_consts* k, int interp, int cf_mode, do
         double vz = (double)k->min_z + (double)iz * (double)k->res_z;
         double ar = 0.0, ai = 0.0, inc = 0.0;
         long long act = 0;
-        if (hercules) {
+        if (k->sequence == UFORCES) {
+            uforces_voxel(iq, k, vx, vy, vz, interp, &ar, &ai, &inc, &act);
+        } else if (k->sequence == RCA_TPW) {
+            tpw_voxel(iq, k, vx, vy, vz, interp, &ar, &ai, &inc, &act);
+        } else if (hercules) {
             double fz = k->focal_z;
             double tx_y = 0.0, tx_z = vz; /* YZ_PLANE with zero steering */
             if (k->focus_type == 4) { tx_y = vy; tx_z = vz - fz; }

@@ -53,14 +53,18 @@ typedef struct or_das_consts {
     int readi_groups, readi_id, encoding;
     int apo_type, to_power;
     int sequence;
+    /* per-transmit geometry of the sequences of beamformer_kernels.cu */
+    float fs, fc;                       /* sampling and centre frequency [Hz] */
+    float tx_angle[MAX_ACQ_COUNT];      /* RCA_TPW: steering angle of transmit t = foci[3t+2] */
+    int tx_element[MAX_ACQ_COUNT];      /* UFORCES: sparse_elements[t] */
 } or_das_consts;
 
 int or_das_constants(const CudaBeamformerParameters* bp, or_das_consts* k);
 /* interp: 0 = Catmull-Rom cubic (reference), 1 = linear.  cf_mode: 0 = reference
- * behaviour (FORCES applies the coherence factor, HERCULES does not), 1 = apply
- * it for HERCULES as well.  Returns the number of taps with apo above the
- * sequence's threshold through *active (may be NULL).  out: interleaved complex
- * [Nz][Ny][Nx]. */
+ * behaviour (FORCES, UFORCES and RCA_TPW apply the coherence factor, HERCULES
+ * does not), 1 = apply it for HERCULES as well.  Returns the number of taps
+ * with apo above the sequence's threshold through *active (may be NULL).  out:
+ * interleaved complex [Nz][Ny][Nx]. */
 int or_das(const double* iq, const or_das_consts* k, int interp, int cf_mode, double* out, long long* active);
 int or_readi_row_bits(const or_das_consts* k, uint64_t* bits);
 
