@@ -121,9 +121,10 @@ def motion_leg(dev, stream, seed=20261203, repeats=5):
     return out
 
 
-def legacy_leg(dev, stream, repeats=3):
-    """UFORCES (2-D, 32-element linear array) and RCA_TPW (3-D, 32 tilted
-    plane waves x 128 columns -> 128^3) DAS kernels on device-resident IQ."""
+def sequences_leg(dev, stream, repeats=3):
+    """FORCES and UFORCES (2-D, 32-element linear array) and RCA_TPW (3-D, 32
+    tilted plane waves x 128 columns -> 128^3) DAS kernels on device-resident
+    IQ."""
     import ctypes as C
     import torch
 
@@ -131,6 +132,8 @@ def legacy_leg(dev, stream, repeats=3):
     lib = L.load()
     out = {}
     cases = {
+        "forces_2d": (configs.linear_2d(seed=1, generate=False),
+                      "FORCES: 32 tx x 32 ch x 1024 samples -> 128 x 256, cubic, no apodization"),
         "uforces_2d": (configs.uforces_2d(seed=6, T=32, S=1024, generate=False),
                        "UFORCES: flash + 31 sparse transmits x 32 ch x 1024 samples -> 128 x 256, cubic"),
         "tpw_3d": (configs.tpw_3d(seed=2, T=32, C=128, S=4096, n=128, generate=False, z_range=(5e-3, 60e-3),
@@ -158,7 +161,7 @@ def legacy_leg(dev, stream, repeats=3):
         out[name] = {"workload": desc, "ms": round(ms, 4), "active_taps": taps.value,
                      "gtaps_per_s": round(taps.value / (ms * 1e-3) / 1e9, 2)}
         del d_iq, d_vol
-    log(f"bench: legacy sequences {json.dumps(out)}")
+    log(f"bench: sequences {json.dumps(out)}")
     return out
 
 
@@ -220,7 +223,7 @@ def main():
     ap.add_argument("--no-motion", action="store_true", help="skip the 3-D motion leg (learned vs full search)")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-frame uneven batch leg")
     ap.add_argument("--batch-frames", type=int, default=64)
-    ap.add_argument("--no-legacy", action="store_true", help="skip the UFORCES / RCA_TPW kernel timings")
+    ap.add_argument("--no-sequences", action="store_true", help="skip the FORCES / UFORCES / RCA_TPW kernel timings")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one device-to-device copy of the 512 MiB IQ buffer (counter calibration)")
     args = ap.parse_args()
@@ -318,7 +321,7 @@ def main():
         torch.cuda.synchronize()
         log(f"bench: calibration copy of {d_iq.numel() * 4} bytes")
     motion = None if args.no_motion else motion_leg(dev, stream)
-    legacy = None if args.no_legacy else legacy_leg(dev, stream)
+    sequences = None if args.no_sequences else sequences_leg(dev, stream)
     batch = None if args.no_batch else batch_leg(dev, args.batch_frames)
     api.set_stream(None)
 
@@ -372,7 +375,7 @@ def main():
         "precision_cf_on": precision,
         "motion": motion,
         "batch": batch,
-        "legacy_sequences": legacy,
+        "das_sequences": sequences,
     }
     print(json.dumps(out), flush=True)
 

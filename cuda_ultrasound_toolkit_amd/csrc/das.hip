@@ -496,24 +496,100 @@ __global__ __launch_bounds__(256) void hercules_reduce(const float4* __restrict_
     vol[v] = make_float2(ax, ay);
 }
 
-// FORCES: one thread per voxel (the volume is y-invariant in the reference's
-// arithmetic; y only selects the output row).
-template <bool CUBIC>
-__global__ __launch_bounds__(256) void forces_kernel(const void* __restrict__ iq, const DasParams p,
-                                                     float2* __restrict__ vol)
+// The per-voxel kernels below (FORCES, UFORCES, RCA_TPW) work on groups of
+// 64 voxels (one per lane).  A group's sum is split over nw waves of one
+// workgroup (an outer loop of the sum: wave w takes iterations w, w + nw,
+// ...), so that small 2-D grids still fill the chip; the partial sums meet
+// in LDS and are added in wave order (deterministic).  A workgroup has
+// max(4, nw) waves, i.e. 4 / nw groups when nw < 4.
+constexpr int VOX_MAX_WAVES = 16;
+
+inline unsigned vox_block(int nw) { return 64u * (unsigned)(nw < 4 ? 4 : nw); }
+inline unsigned vox_grid(long long nvox, int nw)
 {
-    const long long nvox = (long long)p.Nx * p.Ny * p.Nz;
-    const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= nvox) return;
-    const int ix = (int)(v % p.Nx);
-    const int iz = (int)(v / ((long long)p.Nx * p.Ny));
+    const long long groups = (nvox + 63) / 64, per_block = nw < 4 ? 4 / nw : 1;
+    return (unsigned)((groups + per_block - 1) / per_block);
+}
+
+// Calls f(std::integral_constant<int, nw>) for nw in {1, 2, 4, 8, 16}.
+template <class F>
+void with_waves(int nw, F&& f)
+{
+    switch (nw) {
+        case 1: f(std::integral_constant<int, 1>{}); break;
+        case 2: f(std::integral_constant<int, 2>{}); break;
+        case 4: f(std::integral_constant<int, 4>{}); break;
+        case 8: f(std::integral_constant<int, 8>{}); break;
+        default: f(std::integral_constant<int, 16>{}); break;
+    }
+}
+
+struct VoxelLane {
+    long long v;  // voxel (clamped to the grid; `valid` says whether it is stored)
+    bool valid;
+    int w, nw;    // wave within the voxel group, waves per voxel group
+};
+
+__device__ __forceinline__ VoxelLane voxel_lane(long long nvox, int nw)
+{
+    VoxelLane r;
+    const int wb = threadIdx.x >> 6;  // wave of the workgroup
+    const int per_block = (int)(blockDim.x >> 6) / nw;
+    r.v = ((long long)blockIdx.x * per_block + wb / nw) * 64 + (threadIdx.x & 63);
+    r.valid = r.v < nvox;
+    if (!r.valid) r.v = nvox - 1;
+    r.w = wb % nw;
+    r.nw = nw;
+    return r;
+}
+
+__device__ __forceinline__ float2 apply_cf(float2 acc, float inc, float cw)
+{
+    float cf = powf((acc.x * acc.x + acc.y * acc.y) / inc, cw);
+    if (isnan(cf)) cf = 0.0f;
+    return make_float2(acc.x * cf, acc.y * cf);
+}
+
+// Adds the waves' partials of each voxel (wave 0, 1, ... in order), applies
+// the coherence factor and stores.  Every thread of the workgroup calls it.
+__device__ __forceinline__ void store_voxel(const VoxelLane& vl, float2 acc, float inc, float cw,
+                                            float2* __restrict__ vol)
+{
+    __shared__ float4 part[VOX_MAX_WAVES * 64];
+    if (vl.nw > 1) {
+        const int l = threadIdx.x & 63, wb = threadIdx.x >> 6;
+        part[wb * 64 + l] = make_float4(acc.x, acc.y, inc, 0.0f);
+        __syncthreads();
+        if (vl.w != 0) return;
+        acc = make_float2(0.0f, 0.0f);
+        inc = 0.0f;
+        for (int k = 0; k < vl.nw; ++k) {
+            const float4 q = part[(wb + k) * 64 + l];
+            acc.x += q.x;
+            acc.y += q.y;
+            inc += q.z;
+        }
+    }
+    if (vl.valid) vol[vl.v] = apply_cf(acc, inc, cw);
+}
+
+// FORCES (templated_beamformer.cuh:219-324): the volume is y-invariant in the
+// reference's arithmetic (y only selects the output row).  The waves split
+// the receive-channel loop.
+template <bool CUBIC, int NW>
+__global__ __launch_bounds__(1024) void forces_kernel(const void* __restrict__ iq, const DasParams p,
+                                                      float2* __restrict__ vol)
+{
+    const VoxelLane vl = voxel_lane((long long)p.Nx * p.Ny * p.Nz, NW);
+    const int ix = (int)(vl.v % p.Nx);
+    const int iz = (int)(vl.v / ((long long)p.Nx * p.Ny));
     const float vx = p.min_x + (float)ix * p.res_x;
     const float vz = p.min_z + (float)iz * p.res_z;
     const int G = p.readi_groups > 1 ? p.readi_groups : 1;
     float2 acc = make_float2(0.0f, 0.0f);
     float inc = 0.0f;
     const float vz2 = vz * vz;
-    for (int c = 0; c < p.C; ++c) {
+    for (int c = vl.w; c < p.C; c += NW) {
         const float ec = fmaf((float)c, p.pitch_x, p.xdc_min_x);
         const float rx = ec - vx;
         const float rxd = fsqrt(fmaf(rx, rx, vz2));
@@ -540,9 +616,7 @@ __global__ __launch_bounds__(256) void forces_kernel(const void* __restrict__ iq
             }
         }
     }
-    float cf = powf((acc.x * acc.x + acc.y * acc.y) / inc, p.cw);
-    if (isnan(cf)) cf = 0.0f;
-    vol[v] = make_float2(acc.x * cf, acc.y * cf);
+    store_voxel(vl, acc, inc, p.cw, vol);
 }
 
 // ---------------------------------------------------------------------------
@@ -555,12 +629,6 @@ __global__ __launch_bounds__(256) void forces_kernel(const void* __restrict__ iq
 // Weight of the first used UFORCES transmit (beamformer_kernels.cu:148-151).
 constexpr float UF_FIRST_SCALE = 0.088388347648318f;
 
-__device__ __forceinline__ float2 apply_cf(float2 acc, float inc, float cw)
-{
-    float cf = powf((acc.x * acc.x + acc.y * acc.y) / inc, cw);
-    if (isnan(cf)) cf = 0.0f;
-    return make_float2(acc.x * cf, acc.y * cf);
-}
 
 // UFORCES: data row block t + 1 (block 0 is a discarded flash) was fired from
 // element sparse_elements[t].  The transmit vector is calc_tx_distance(vox,
@@ -572,14 +640,13 @@ __device__ __forceinline__ float2 apply_cf(float2 acc, float inc, float cw)
 // Precision as in the RCA_TPW kernel below: per voxel and transmit the
 // channel-independent part (z + |tx| sign) spm + delay is FP64 and split
 // into an integer and a fraction; per channel only (|rx| - z) spm =
-// rx_x^2 / (|rx| + z) spm is FP32.
-template <bool CUBIC>
-__global__ __launch_bounds__(256) void uforces_kernel(const float2* __restrict__ iq, const DasParams p,
-                                                      float2* __restrict__ vol)
+// rx_x^2 / (|rx| + z) spm is FP32.  The waves split the transmit loop.
+template <bool CUBIC, int NW>
+__global__ __launch_bounds__(1024) void uforces_kernel(const float2* __restrict__ iq, const DasParams p,
+                                                       float2* __restrict__ vol)
 {
-    const long long nvox = (long long)p.Nx * p.Ny * p.Nz;
-    const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= nvox) return;
+    const VoxelLane vl = voxel_lane((long long)p.Nx * p.Ny * p.Nz, NW);
+    const long long v = vl.v;
     const int ix = (int)(v % p.Nx), iy = (int)((v / p.Nx) % p.Ny), iz = (int)(v / ((long long)p.Nx * p.Ny));
     const double vxd = (double)p.min_x + ix * (double)p.res_x;
     const double vyd = (double)p.min_y + iy * (double)p.res_y;
@@ -605,7 +672,7 @@ __global__ __launch_bounds__(256) void uforces_kernel(const float2* __restrict__
     const int n = p.T * p.C * p.S;
     float2 acc = make_float2(0.0f, 0.0f);
     float inc = 0.0f;
-    for (int t = 0; t + 1 < p.T; ++t) {
+    for (int t = vl.w; t + 1 < p.T; t += NW) {
         const double txx = fma((double)p.tx_tab[t].x, (double)p.pitch_x, tx0);
         const double X0 = (vzd + sqrt(fma(txx, txx, tyz2)) * sgn) * (double)p.spm + (double)p.delay;
         const double I0 = fmin(fmax(floor(X0), -1048576.0), 1048576.0);
@@ -639,7 +706,7 @@ __global__ __launch_bounds__(256) void uforces_kernel(const float2* __restrict__
             inc = fmaf(qx, qx, fmaf(qy, qy, inc));
         }
     }
-    vol[v] = apply_cf(acc, inc, p.cw);
+    store_voxel(vl, acc, inc, p.cw, vol);
 }
 
 // Steering table of RCA_TPW in FP64: (sin, cos) of each transmit's angle.
@@ -667,14 +734,13 @@ constexpr int TPW_QB = 8;  // angles per register block
 // (|rx| - z) spm = rx_x^2 / (|rx| + z) spm (at most a few hundred samples),
 // is FP32.  The tap's index is I0 + (fraction + excess), the clamp bounds are
 // moved by I0 (integers, exact), and the rotation's phase is frac(I0 fc/fs)
-// (FP64) + (fraction + excess) fc/fs.
-template <bool CUBIC>
-__global__ __launch_bounds__(256) void tpw_kernel(const float2* __restrict__ iq, const DasParams p,
-                                                  const double2* __restrict__ sc, float2* __restrict__ vol)
+// (FP64) + (fraction + excess) fc/fs.  The waves split the channel loop.
+template <bool CUBIC, int NW>
+__global__ __launch_bounds__(1024) void tpw_kernel(const float2* __restrict__ iq, const DasParams p,
+                                                   const double2* __restrict__ sc, float2* __restrict__ vol)
 {
-    const long long nvox = (long long)p.Nx * p.Ny * p.Nz;
-    const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= nvox) return;
+    const VoxelLane vl = voxel_lane((long long)p.Nx * p.Ny * p.Nz, NW);
+    const long long v = vl.v;
     const int ix = (int)(v % p.Nx), iy = (int)((v / p.Nx) % p.Ny), iz = (int)(v / ((long long)p.Nx * p.Ny));
     const double vxd = (double)p.min_x + ix * (double)p.res_x;
     const double vyd = (double)p.min_y + iy * (double)p.res_y;
@@ -706,7 +772,7 @@ __global__ __launch_bounds__(256) void tpw_kernel(const float2* __restrict__ iq,
             i0[j] = (int)I0;
             row[j] = q * p.C * p.S;
         }
-        for (int c = 0; c < p.C; ++c) {
+        for (int c = vl.w; c < p.C; c += NW) {
             const float rxx = fmaf((float)c, p.pitch_x, rx0);
             const float a = p.fn_rx * (fabsf(rxx) / vz) * 0.5f;
             // cos^2(pi min(a, 1/2)) >= 0.1  <=>  a < acos(sqrt(0.1)) / pi
@@ -738,21 +804,79 @@ __global__ __launch_bounds__(256) void tpw_kernel(const float2* __restrict__ iq,
             }
         }
     }
-    vol[v] = apply_cf(acc, inc, p.cw);
+    store_voxel(vl, acc, inc, p.cw, vol);
 }
 
-// Transmit-split tuning of the single-frame HERCULES launch: the heuristic
-// das_splits, or, with autotuning on, the fastest of 1, 2, 4 and 8 splits,
-// measured the first time a shape is seen (outside stream capture).  Every
-// candidate computes the whole volume; only the order in which the per-split
-// partial sums are added differs.
+// Launch tuning: the transmit split of the single-frame HERCULES launch (the
+// heuristic das_splits, or the fastest of 1, 2, 4 and 8 splits) and the waves
+// per voxel group of the per-voxel kernels (vox_waves, or the fastest of 1 to
+// 16), measured the first time a shape is seen (outside stream capture).
 namespace {
+using TuneKey = std::array<long long, 12>;
 std::mutex g_tune_mutex;
-std::map<std::array<long long, 12>, int> g_tuned_splits;
+std::map<TuneKey, int> g_tuned_splits;  // HERCULES transmit splits
+std::map<TuneKey, int> g_tuned_waves;   // waves per voxel group of the per-voxel kernels
 bool g_das_autotune = true;
 constexpr int TUNE_MAX_SPLITS = 8;
 
 int max_splits(int gt_total) { return std::max(1, std::min(TUNE_MAX_SPLITS, gt_total / 8)); }
+
+// Waves per 64-voxel group: enough for ~8192 waves in flight (32 per CU),
+// at most VOX_MAX_WAVES.
+int vox_waves(long long nvox)
+{
+    const long long groups = (nvox + 63) / 64;
+    int nw = 1;
+    while (nw < VOX_MAX_WAVES && groups * nw < 8192) nw *= 2;
+    return nw;
+}
+
+// Runs run(choice) with the choice tuned for `key`, else the default; with
+// autotuning on and the stream not capturing, the first call of a key runs
+// the default once untimed (first-launch costs), then times every candidate
+// (1, 2, 4, ... up to max_cand) twice and keeps the fastest.  Every candidate
+// computes the complete result (only the summation order differs), so the
+// last timed run's output stands.
+template <class Run>
+bool tuned_run(std::map<TuneKey, int>& table, const TuneKey& key, int dflt, int max_cand, hipStream_t st, Run&& run)
+{
+    int choice = dflt;
+    bool tune = false;
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mutex);
+        auto it = table.find(key);
+        if (it != table.end()) choice = it->second;
+        else tune = g_das_autotune && max_cand > 1;
+    }
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (!tune || hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return run(choice);
+    if (!run(dflt)) return false;
+    hipEvent_t a, b;
+    CUT_HIP_CHECK(hipEventCreate(&a));
+    CUT_HIP_CHECK(hipEventCreate(&b));
+    float best = 1e30f;
+    int best_c = dflt;
+    for (int k = 0; k < 2 * 16 && (1 << (k >> 1)) <= max_cand; ++k) {
+        const int cand = 1 << (k >> 1);
+        float ms = 0.0f;
+        const bool ok = hipEventRecord(a, st) == hipSuccess && run(cand) && hipEventRecord(b, st) == hipSuccess &&
+                        hipEventSynchronize(b) == hipSuccess && hipEventElapsedTime(&ms, a, b) == hipSuccess;
+        if (!ok) {
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+            CUT_FAIL("DAS: launch tuning failed");
+        }
+        if (ms < best) {
+            best = ms;
+            best_c = cand;
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    table[key] = best_c;
+    return true;
+}
 }  // namespace
 
 void set_das_autotune(bool on)
@@ -919,15 +1043,16 @@ bool launch_das(const void* iq, int iq_fmt, float iq_scale, const DasParams& p, 
         const bool focus = p.focus_type == 4;
         const bool sinapo = p.apo_type == RX_TO_SIN;
         const bool cf = o.cf_hercules != 0;
+        const int dflt = das_splits(tiles, G * p.T);
+        auto fits = [&](int sp) {
+            return sp <= 1 || (workspace && workspace_bytes >= (size_t)sp * nvox * sizeof(float4));
+        };
         auto run = [&](int splits) -> bool {
+            // a choice tuned with a larger workspace falls back to one that fits
+            if (!fits(splits)) splits = fits(dflt) ? dflt : 1;
             hg.splits = splits;
             hg.gt_per_split = (G * p.T + splits - 1) / splits;
-            float4* part = nullptr;
-            if (splits > 1) {
-                if (!workspace || workspace_bytes < (size_t)splits * nvox * sizeof(float4))
-                    CUT_FAIL("DAS: workspace of " << (size_t)splits * nvox * sizeof(float4) << " bytes needed");
-                part = (float4*)workspace;
-            }
+            float4* part = splits > 1 ? (float4*)workspace : nullptr;
             if (tiles * splits > 0x7fffffffLL) CUT_FAIL("DAS: grid too large");
             dim3 grid((unsigned)(tiles * splits));
             const bool ok = o.cubic ? launch_herc_fo<true>(focus, cf, sinapo, iq_fmt, grid, st, iq, inv, p, hg, vol, part)
@@ -938,51 +1063,24 @@ bool launch_das(const void* iq, int iq_fmt, float iq_scale, const DasParams& p, 
                 hercules_reduce<<<(unsigned)((nvox + 255) / 256), 256, 0, st>>>(part, splits, nvox, cf ? 1 : 0, p.cw, vol);
             return true;
         };
-        int splits = das_splits(tiles, G * p.T);
-        const std::array<long long, 12> key{p.Nx, p.Ny, p.Nz, p.T, p.C, p.S, G, iq_fmt, o.cubic, cf, focus, sinapo};
-        bool tune = false;
-        {
-            std::lock_guard<std::mutex> lock(g_tune_mutex);
-            auto it = g_tuned_splits.find(key);
-            if (it != g_tuned_splits.end()) splits = it->second;
-            else tune = g_das_autotune;
-        }
-        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-        if (tune && hipStreamIsCapturing(st, &cs) == hipSuccess && cs == hipStreamCaptureStatusNone &&
-            (size_t)max_splits(G * p.T) * nvox * sizeof(float4) <= workspace_bytes) {
-            hipEvent_t a, b;
-            CUT_HIP_CHECK(hipEventCreate(&a));
-            CUT_HIP_CHECK(hipEventCreate(&b));
-            float best = 1e30f;
-            int best_s = splits;
-            for (int cand = 1; cand <= max_splits(G * p.T); cand *= 2) {
-                float ms = 0.0f;
-                const bool ok = hipEventRecord(a, st) == hipSuccess && run(cand) && hipEventRecord(b, st) == hipSuccess &&
-                                hipEventSynchronize(b) == hipSuccess && hipEventElapsedTime(&ms, a, b) == hipSuccess;
-                if (!ok) {
-                    (void)hipEventDestroy(a);
-                    (void)hipEventDestroy(b);
-                    CUT_FAIL("DAS: split tuning failed");
-                }
-                if (ms < best) {
-                    best = ms;
-                    best_s = cand;
-                }
-            }
-            (void)hipEventDestroy(a);
-            (void)hipEventDestroy(b);
-            std::lock_guard<std::mutex> lock(g_tune_mutex);
-            g_tuned_splits[key] = best_s;
-            return true;  // the last candidate left a complete volume
-        }
-        if (!run(splits)) return false;
+        const TuneKey key{p.Nx, p.Ny, p.Nz, p.T, p.C, p.S, G, iq_fmt, o.cubic, cf, focus, sinapo};
+        const int cand = fits(max_splits(G * p.T)) ? max_splits(G * p.T) : 1;
+        if (!tuned_run(g_tuned_splits, key, dflt, cand, st, run)) return false;
     } else if (p.sequence == FORCES) {
         if (p.readi_groups > 1 && p.encoding != HADAMARD) CUT_FAIL("FORCES only supports Hadamard encoding at this time.");
         if (iq_fmt != IQ_F32) CUT_FAIL("DAS: staged IQ formats are supported for HERCULES only");
         const long long nvox = (long long)p.Nx * p.Ny * p.Nz;
-        dim3 grid((unsigned)((nvox + 255) / 256));
-        if (o.cubic) forces_kernel<true><<<grid, 256, 0, st>>>(iq, p, vol);
-        else forces_kernel<false><<<grid, 256, 0, st>>>(iq, p, vol);
+        auto run = [&](int nw) -> bool {
+            const unsigned grid = vox_grid(nvox, nw), block = vox_block(nw);
+            with_waves(nw, [&](auto w) {
+                if (o.cubic) forces_kernel<true, w()><<<grid, block, 0, st>>>(iq, p, vol);
+                else forces_kernel<false, w()><<<grid, block, 0, st>>>(iq, p, vol);
+            });
+            CUT_HIP_CHECK(hipGetLastError());
+            return true;
+        };
+        const TuneKey key{FORCES, p.Nx, p.Ny, p.Nz, p.T, p.C, p.S, p.readi_groups, o.cubic, p.apo_type, 0, 0};
+        if (!tuned_run(g_tuned_waves, key, vox_waves(nvox), VOX_MAX_WAVES, st, run)) return false;
     } else if (p.sequence == UFORCES || p.sequence == RCA_TPW) {
         if (iq_fmt != IQ_F32) CUT_FAIL("DAS: staged IQ formats are supported for HERCULES only");
         if (p.T > MAX_ACQ_COUNT) CUT_FAIL("DAS: at most " << MAX_ACQ_COUNT << " transmits");
@@ -991,20 +1089,31 @@ bool launch_das(const void* iq, int iq_fmt, float iq_scale, const DasParams& p, 
             CUT_FAIL("TPW beamforming requires a plane focus.");
         if (p.sequence == UFORCES && p.focus_type == 0) CUT_FAIL("Beamformer: UFORCES needs a transmit orientation.");
         const long long nvox = (long long)p.Nx * p.Ny * p.Nz;
-        dim3 grid((unsigned)((nvox + 255) / 256));
         const float2* in = static_cast<const float2*>(iq);
-        if (p.sequence == UFORCES) {
-            if (o.cubic) uforces_kernel<true><<<grid, 256, 0, st>>>(in, p, vol);
-            else uforces_kernel<false><<<grid, 256, 0, st>>>(in, p, vol);
-        } else {
+        double2* sc = nullptr;
+        if (p.sequence == RCA_TPW) {
             if (!workspace || workspace_bytes < (size_t)p.T * sizeof(double2))
                 CUT_FAIL("DAS: workspace of " << (size_t)p.T * sizeof(double2) << " bytes needed");
-            double2* sc = static_cast<double2*>(workspace);
+            sc = static_cast<double2*>(workspace);
             tpw_table_kernel<<<1, 256, 0, st>>>(p, sc);
-            if (o.cubic) tpw_kernel<true><<<grid, 256, 0, st>>>(in, p, sc, vol);
-            else tpw_kernel<false><<<grid, 256, 0, st>>>(in, p, sc, vol);
+            CUT_HIP_CHECK(hipGetLastError());
         }
-        CUT_HIP_CHECK(hipGetLastError());
+        auto run = [&](int nw) -> bool {
+            const unsigned grid = vox_grid(nvox, nw), block = vox_block(nw);
+            with_waves(nw, [&](auto w) {
+                if (p.sequence == UFORCES) {
+                    if (o.cubic) uforces_kernel<true, w()><<<grid, block, 0, st>>>(in, p, vol);
+                    else uforces_kernel<false, w()><<<grid, block, 0, st>>>(in, p, vol);
+                } else {
+                    if (o.cubic) tpw_kernel<true, w()><<<grid, block, 0, st>>>(in, p, sc, vol);
+                    else tpw_kernel<false, w()><<<grid, block, 0, st>>>(in, p, sc, vol);
+                }
+            });
+            CUT_HIP_CHECK(hipGetLastError());
+            return true;
+        };
+        const TuneKey key{p.sequence, p.Nx, p.Ny, p.Nz, p.T, p.C, p.S, p.focus_type, o.cubic, 0, 0, 0};
+        if (!tuned_run(g_tuned_waves, key, vox_waves(nvox), VOX_MAX_WAVES, st, run)) return false;
     } else {
         CUT_FAIL("Beamformer: Unsupported sequence ID " << p.sequence);
     }

@@ -68,10 +68,11 @@ typedef struct CutFrame {
  * from the start of the batch on the GPU to that frame's completion. */
 CUT_EXT_EXPORT bool cut_beamform_batch(const CutFrame* frames, int n, int mode, double* frame_ms);
 /* Autotuning, on by default: the batch DAS launch shape (grid size, work
- * order) per batch shape, and the single-frame HERCULES transmit split (1, 2,
- * 4 or 8 workgroups per tile, partial sums added in split order) per frame
- * shape; each is timed on first use (that call synchronises) and the fastest
- * kept. */
+ * order) per batch shape, the single-frame HERCULES transmit split (1, 2, 4
+ * or 8 workgroups per tile, partial sums added in split order) and, for
+ * FORCES, UFORCES and RCA_TPW, the waves per 64-voxel workgroup (1 to 16,
+ * each taking a share of the voxel's outer loop) per frame shape; each is
+ * timed on first use (that call synchronises) and the fastest kept. */
 CUT_EXT_EXPORT bool cut_set_autotune(int on);
 /* Launch shape the last graph-mode batch used: resident workgroups per CU
  * (0 = one workgroup per tile) and work order (0 heaviest first, 1 frame by frame). */
