@@ -124,7 +124,7 @@ def motion_leg(dev, stream, seed=20261203, repeats=5):
 def sequences_leg(dev, stream, repeats=3):
     """FORCES and UFORCES (2-D, 32-element linear array) and RCA_TPW (3-D, 32
     tilted plane waves x 128 columns -> 128^3) DAS kernels on device-resident
-    IQ."""
+    IQ, and the bandpass input stage at the full frame size."""
     import ctypes as C
     import torch
 
@@ -161,6 +161,32 @@ def sequences_leg(dev, stream, repeats=3):
         out[name] = {"workload": desc, "ms": round(ms, 4), "active_taps": taps.value,
                      "gtaps_per_s": round(taps.value / (ms * 1e-3) / 1e9, 2)}
         del d_iq, d_vol
+    # bandpass input stage at the full frame size: (I, Q) int16 pairs -> demodulated complex, 64-tap FIR
+    import numpy as np
+    T, Cc, S, taps_n = 128, 128, 4096, 64
+    g = torch.Generator(device=dev).manual_seed(12)
+    d_raw = (torch.randn(Cc * 2 * T * S, device=dev, generator=g) * 3000).to(torch.int16)
+    d_out = torch.empty(T * Cc * S * 2, dtype=torch.float32, device=dev)
+    assert api.rf_init((2 * T * S, Cc), (S, Cc, T), L.NONE)
+    assert api.rf_set_channel_mapping(np.arange(Cc, dtype=np.int16))
+    assert api.rf_set_match_filter(np.hanning(taps_n + 2)[1:-1].astype(np.float32))
+    run = lambda: api.rf_convert_demod(d_raw, L.TYPE_I16, d_out, 5e6, 20e6)
+    assert run()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    for _ in range(repeats):
+        assert run()
+    b.record(stream)
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / repeats
+    nbytes = (4 + 8) * T * Cc * S
+    out["bandpass_demod"] = {"workload": "bandpass (I, Q) int16 pairs 128 tx x 128 ch x 4096 -> demodulated complex, "
+                                         "64-tap centred FIR", "ms": round(ms, 4),
+                             "GBs": round(nbytes / (ms * 1e-3) / 1e9, 1),
+                             "frac_hbm": round(nbytes / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
+    assert api.rf_set_match_filter(None)
+    del d_raw, d_out
     log(f"bench: sequences {json.dumps(out)}")
     return out
 

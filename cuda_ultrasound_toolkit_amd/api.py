@@ -153,6 +153,12 @@ def rf_convert_decode(d_raw, dtype: int, d_out, out_stride: int = 1, decode: boo
     return bool(L.load().cut_rf_convert_decode(d_raw.data_ptr(), dtype, d_out.data_ptr(), out_stride, int(decode)))
 
 
+def rf_convert_demod(d_raw, dtype: int, d_out, demod_freq: float, sample_freq: float) -> bool:
+    """Bandpass (I, Q)-pair rows -> demodulated, FIR-filtered complex [T][C][S] (device tensors)."""
+    return bool(L.load().cut_rf_convert_demod(d_raw.data_ptr(), dtype, d_out.data_ptr(), float(demod_freq),
+                                              float(sample_freq)))
+
+
 def rf_hilbert(d_in, in_stride: int, d_out) -> bool:
     return bool(L.load().cut_rf_hilbert(d_in.data_ptr(), in_stride, d_out.data_ptr()))
 

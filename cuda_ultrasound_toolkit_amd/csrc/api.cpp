@@ -182,6 +182,15 @@ bool cut_rf_convert_decode(const void* d_raw, int type, float* d_out, int out_st
     return stage_state().rf.convert_decode(d_raw, type, d_out, out_stride, decode != 0, cut::stream());
 }
 
+bool cut_rf_convert_demod(const void* d_raw, int type, void* d_out, float demod_freq, float sample_freq)
+{
+    if (!d_raw || !d_out) CUT_FAIL("null argument");
+    if (!(sample_freq > 0.0f)) CUT_FAIL("sample frequency must be positive");
+    std::lock_guard<std::mutex> lock(cut::api_mutex());
+    return stage_state().rf.convert_demod(d_raw, type, (float2*)d_out, (double)demod_freq / (double)sample_freq,
+                                          cut::stream());
+}
+
 bool cut_rf_hilbert(const float* d_in, int in_stride, void* d_out)
 {
     if (!d_in || !d_out) CUT_FAIL("null argument");

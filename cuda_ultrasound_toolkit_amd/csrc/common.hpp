@@ -76,6 +76,13 @@ void set_stream(hipStream_t s);
 bool launch_convert(const void* raw, int type, const int16_t* d_map, size_t R0, int T, int C, int S,
                     float* out, int out_stride, hipStream_t st);
 
+// Bandpass input (data_conversion_kernels.cuh:14-102): (I, Q) pairs at
+// raw[map[c]*R0 + 2*(t*S + s)], v = (I - iQ) exp(-2 pi i s fd/fs), then with
+// L > 0 taps the centred FIR sum_i v[s + i - L/2] h[L-1-i] over the taps
+// inside the row, divided by their count.  out: complex [T][C][S].
+bool launch_convert_demod(const void* raw, int type, const int16_t* d_map, size_t R0, int T, int C, int S,
+                          double fd_fs, const float* d_h, int L, float2* out, hipStream_t st);
+
 // Decode (hadamard_decoder.cpp:49-78): out[j][c][s] = scale * sum_k H[j][k] raw[map[c]][k*S+s].
 // mode 0: FP32 MFMA GEMM (any input type); mode 1: int8 MFMA on hi/lo byte
 // splits of int16 input with int32 accumulation.  d_h8: int8 H padded to

@@ -193,6 +193,17 @@ bool RfPipeline::deinit()
     return true;
 }
 
+bool RfPipeline::convert_demod(const void* d_raw, int type, float2* d_out, double fd_fs, hipStream_t st)
+{
+    if (!init_) CUT_FAIL("Session not initialized.");
+    if (!check_map(RC_)) return false;
+    if ((size_t)R0_ < 2 * (size_t)T_ * S_)
+        CUT_FAIL("rf_raw_dim[0] = " << R0_ << " is shorter than 2*transmits*samples for (I, Q) pairs.");
+    const int L = filter_on_ ? (int)filter_host_.size() : 0;
+    return launch_convert_demod(d_raw, type, map_.as<int16_t>(), R0_, (int)T_, (int)C_, (int)S_, fd_fs,
+                                filter_taps_.as<float>(), L, d_out, st);
+}
+
 bool RfPipeline::set_channel_mapping(const int16_t* map, size_t count)
 {
     if (!map || count == 0) CUT_FAIL("Failed to load channel mapping: empty map");
@@ -457,12 +468,22 @@ bool validate_beamform(const CudaBeamformerParameters& bp, size_t raw_bytes, siz
 {
     const uint R0 = bp.rf_raw_dim[0], RC = bp.rf_raw_dim[1];
     const uint S = bp.dec_data_dim[0], C = bp.dec_data_dim[1], T = bp.dec_data_dim[2];
-    if (bp.sample_type != SAMPLE_NORMAL) CUT_FAIL("Unsupported sample type: " << bp.sample_type);
+    const bool bandpass = bp.sample_type == SAMPLE_BANDPASS;
+    if (bp.sample_type != SAMPLE_NORMAL && !bandpass) CUT_FAIL("Unsupported sample type: " << bp.sample_type);
     if (bp.data_type != TYPE_I16 && bp.data_type != TYPE_F32) CUT_FAIL("Data converter: Unsupported input data type.");
     if (S == 0 || C == 0 || T == 0) CUT_FAIL("Invalid dimensions for RF raw data: " << S << " samples, " << C * T << " channels.");
     if (C > MAX_CHANNEL_COUNT) CUT_FAIL("Channel count " << C << " exceeds " << MAX_CHANNEL_COUNT << ".");
-    if (!hilbert_supported((int)S)) CUT_FAIL("Sample count " << S << " is not supported by the FFT.");
-    if ((size_t)R0 < (size_t)T * S) CUT_FAIL("rf_raw_dim[0] = " << R0 << " is shorter than transmits*samples.");
+    if (bandpass) {
+        // (I, Q) pairs, demodulated and FIR-filtered; no Hilbert transform, no decode
+        if (bp.decode != NONE) CUT_FAIL("Bandpass input takes no transmit decoding (decode must be NONE).");
+        if ((size_t)R0 < 2 * (size_t)T * S)
+            CUT_FAIL("rf_raw_dim[0] = " << R0 << " is shorter than 2*transmits*samples for (I, Q) pairs.");
+        if (!(bp.sampling_frequency > 0.0f)) CUT_FAIL("Bandpass input needs a positive sampling frequency.");
+        if (options().iq_format != IQ_F32) CUT_FAIL("Staged IQ formats are not available for bandpass input.");
+    } else {
+        if (!hilbert_supported((int)S)) CUT_FAIL("Sample count " << S << " is not supported by the FFT.");
+        if ((size_t)R0 < (size_t)T * S) CUT_FAIL("rf_raw_dim[0] = " << R0 << " is shorter than transmits*samples.");
+    }
     const size_t elem = bp.data_type == TYPE_I16 ? 2 : 4;
     if (raw_bytes < (size_t)R0 * RC * elem)
         CUT_FAIL("Input holds " << raw_bytes << " bytes, rf_raw_dim needs " << (size_t)R0 * RC * elem << ".");
@@ -526,7 +547,11 @@ bool frame_to_iq(RfPipeline& rf, BeamformBuffers& bufs, const void* d_raw, const
 {
     const uint C = bp.dec_data_dim[1];
     void* iq = bufs.iq.ptr;
-    if (bp.decode != NONE) {
+    if (bp.sample_type == SAMPLE_BANDPASS) {
+        const double fd_fs = (double)bp.center_frequency / (double)bp.sampling_frequency;
+        if (!rf.convert_demod(d_raw, bp.data_type, static_cast<float2*>(iq), fd_fs, st))
+            CUT_FAIL("Failed to convert and demodulate data.");
+    } else if (bp.decode != NONE) {
         if (!rf.convert_decode(d_raw, bp.data_type, bufs.dec.as<float>(), 1, true, st)) CUT_FAIL("Failed to decode RF data.");
         HilbertInput in{bufs.dec.ptr, 0, 1, nullptr, 0, (int)C};
         if (!rf.hilbert(in, iq, plan.fmt, plan.scale, st)) CUT_FAIL("Failed to apply Hilbert transform.");

@@ -37,6 +37,10 @@ public:
     bool hilbert(const HilbertInput& in, float2* d_out, hipStream_t st) { return hilbert(in, d_out, IQ_F32, 1.0f, st); }
     // out in IqFormat fmt, holding value * scale
     bool hilbert(const HilbertInput& in, void* d_out, int fmt, float scale, hipStream_t st);
+    // Bandpass input: (I, Q) pairs per sample, demodulated by exp(-2 pi i s
+    // fd/fs) and filtered by the current taps as a centred FIR
+    // (RfProcessor::convert_demod, rf_processing/rf_processor.cpp:225-246)
+    bool convert_demod(const void* d_raw, int type, float2* d_out, double fd_fs, hipStream_t st);
     // Hilbert that converts raw input on the fly (decode == NONE fast path)
     bool convert_hilbert(const void* d_raw, int type, void* d_out, int fmt, float scale, hipStream_t st);
     // Power-of-two scale for FP16-staged IQ of int16 input: the analytic

@@ -54,6 +54,80 @@ bool launch_convert(const void* raw, int type, const int16_t* d_map, size_t R0, 
 }
 
 // ---------------------------------------------------------------------------
+// bandpass input: convert + demodulation + centred FIR in one pass
+// (data_conversion_kernels.cuh:14-21, :46-64, :66-102; data_converter.cu:64-131)
+// ---------------------------------------------------------------------------
+// A workgroup owns BP_SEG outputs of one (t, c) row.  It demodulates the
+// input samples those outputs need (the segment plus the filter's reach on
+// both sides, clipped to the row) into LDS, then each output is the FIR over
+// the valid taps, divided by their count.  The phase s fd/fs is reduced to
+// revolutions in FP64 and evaluated with v_sin/v_cos.
+constexpr int BP_SEG = 1024;
+
+template <typename T>
+__global__ __launch_bounds__(256) void convert_demod_kernel(const T* __restrict__ raw, const int16_t* __restrict__ map,
+                                                            size_t R0, int C, int S, double fd_fs,
+                                                            const float* __restrict__ h, int L,
+                                                            float2* __restrict__ out)
+{
+    __shared__ float2 v[BP_SEG + MAX_FILTER_LENGTH];
+    __shared__ float hr[MAX_FILTER_LENGTH];  // taps in application order: hr[i] = h[L-1-i]
+    const int row = blockIdx.x;              // t * C + c
+    const int t = row / C, c = row - t * C;
+    const int s0 = blockIdx.y * BP_SEG;
+    const int half = L > 0 ? L / 2 : 0;
+    const int lo = max(0, s0 - half);
+    const int hi = min(S - 1, s0 + BP_SEG - 1 + (L > 0 ? L - 1 - half : 0));
+    const T* r = raw + (size_t)(uint16_t)map[c] * R0 + 2 * (size_t)t * S;
+    for (int j = lo + (int)threadIdx.x; j <= hi; j += blockDim.x) {
+        const float I = static_cast<float>(r[2 * (size_t)j]), Q = static_cast<float>(r[2 * (size_t)j + 1]);
+        const double ph = (double)j * fd_fs;
+        const float f = (float)(ph - floor(ph));
+        const float cs = __builtin_amdgcn_cosf(f), sn = __builtin_amdgcn_sinf(f);
+        v[j - lo] = make_float2(I * cs - Q * sn, -I * sn - Q * cs);  // (I - iQ) exp(-2 pi i f)
+    }
+    for (int i = threadIdx.x; i < L; i += blockDim.x) hr[i] = h[L - 1 - i];
+    __syncthreads();
+    for (int k = threadIdx.x; k < BP_SEG; k += blockDim.x) {
+        const int s = s0 + k;
+        if (s >= S) break;
+        float2 y;
+        if (L <= 0) {
+            y = v[s - lo];
+        } else {
+            // taps i with 0 <= s + i - half < S
+            const int ia = max(0, half - s), ib = min(L - 1, S - 1 - s + half);
+            const float2* vs = v + (s - half - lo);
+            float ax = 0.0f, ay = 0.0f;
+            for (int i = ia; i <= ib; ++i) {
+                const float2 q = vs[i];
+                ax = fmaf(q.x, hr[i], ax);
+                ay = fmaf(q.y, hr[i], ay);
+            }
+            const float inv = 1.0f / (float)(ib - ia + 1);
+            y = make_float2(ax * inv, ay * inv);
+        }
+        out[(size_t)row * S + s] = y;
+    }
+}
+
+bool launch_convert_demod(const void* raw, int type, const int16_t* d_map, size_t R0, int T, int C, int S,
+                          double fd_fs, const float* d_h, int L, float2* out, hipStream_t st)
+{
+    if (L < 0 || L > MAX_FILTER_LENGTH || L > S) CUT_FAIL("bandpass filter length " << L << " not in [0, min(S, " << MAX_FILTER_LENGTH << ")]");
+    if (L > 0 && !d_h) CUT_FAIL("bandpass filter taps missing");
+    dim3 grid(T * C, (S + BP_SEG - 1) / BP_SEG);
+    if (type == TYPE_I16)
+        convert_demod_kernel<int16_t><<<grid, 256, 0, st>>>((const int16_t*)raw, d_map, R0, C, S, fd_fs, d_h, L, out);
+    else if (type == TYPE_F32)
+        convert_demod_kernel<float><<<grid, 256, 0, st>>>((const float*)raw, d_map, R0, C, S, fd_fs, d_h, L, out);
+    else
+        CUT_FAIL("Data converter: Unsupported input data type.");
+    CUT_HIP_CHECK(hipGetLastError());
+    return true;
+}
+
+// ---------------------------------------------------------------------------
 // decode
 // ---------------------------------------------------------------------------
 constexpr int DEC_NT = 64;      // samples per workgroup

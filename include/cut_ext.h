@@ -87,6 +87,16 @@ CUT_EXT_EXPORT bool cut_rf_set_match_filter(const float* taps, uint length);
 /* convert (+ decode when decode != 0) into d_out[(t*C+c)*S+s]*out_stride (floats);
  * out_stride 2 writes the real parts of a complex buffer, as the reference does. */
 CUT_EXT_EXPORT bool cut_rf_convert_decode(const void* d_raw, int type, float* d_out, int out_stride, int decode);
+/* Bandpass input (replaces RfProcessor::convert_demod,
+ * cuda_toolkit/src/rf_processing/rf_processor.cpp:225-246, and
+ * DataConverter::convert_and_demod, data_conversion/data_converter.cu:64-131):
+ * row map[c] holds (I, Q) pairs, sample s of transmit t at 2*(t*S + s);
+ * d_out[t][c][s] (complex) = (I - iQ) exp(-2 pi i s demod_freq / sample_freq),
+ * filtered, when match-filter taps are set, by the centred FIR
+ * sum_i v[s + i - L/2] h[L-1-i] over the taps inside the row, divided by
+ * their count.  Needs rf_raw_dim[0] >= 2*T*S. */
+CUT_EXT_EXPORT bool cut_rf_convert_demod(const void* d_raw, int type, void* d_out, float demod_freq,
+                                         float sample_freq);
 /* Hilbert (+ match filter) of rows read at d_in[(row*S+s)*in_stride] -> complex d_out. */
 CUT_EXT_EXPORT bool cut_rf_hilbert(const float* d_in, int in_stride, void* d_out);
 /* DAS of complex IQ [T][C][S] into a complex volume [Nz][Ny][Nx]

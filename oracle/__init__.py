@@ -57,6 +57,8 @@ def load() -> C.CDLL:
     lib.or_hadamard.argtypes = [C.c_int, C.c_int, DP]
     lib.or_decode.argtypes = [DP, DP, C.c_int, C.c_size_t, DP]
     lib.or_convert.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int16), C.c_size_t, C.c_int, C.c_int, C.c_int, DP]
+    lib.or_convert_demod.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int16), C.c_size_t, C.c_int, C.c_int,
+                                     C.c_int, C.c_double, C.c_double, DP, C.c_int, DP]
     lib.or_dft.argtypes = [DP, DP, C.c_int, C.c_int, DP, DP]
     lib.or_fft.argtypes = [DP, DP, C.c_int, C.c_int]
     lib.or_filter_spectrum.argtypes = [DP, C.c_int, C.c_int, DP]
@@ -98,6 +100,22 @@ def convert(raw: np.ndarray, dtype: int, cmap: np.ndarray, R0: int, T: int, Cc: 
     raw = np.ascontiguousarray(raw)
     load().or_convert(raw.ctypes.data, dtype, m.ctypes.data_as(C.POINTER(C.c_int16)), R0, T, Cc, S, _dp(out))
     return out
+
+
+def convert_demod(raw: np.ndarray, dtype: int, cmap: np.ndarray, R0: int, T: int, Cc: int, S: int,
+                  fd: float, fs: float, h=None) -> np.ndarray:
+    """Bandpass (I, Q)-pair rows -> demodulated, optionally FIR-filtered complex [T][C][S]."""
+    out = np.zeros(2 * T * Cc * S, np.float64)
+    m = np.ascontiguousarray(cmap, np.int16)
+    raw = np.ascontiguousarray(raw)
+    if h is None or len(h) == 0:
+        hp, L = None, 0
+    else:
+        hh = np.ascontiguousarray(h, np.float64)
+        hp, L = _dp(hh), len(hh)
+    load().or_convert_demod(raw.ctypes.data, dtype, m.ctypes.data_as(C.POINTER(C.c_int16)), R0, T, Cc, S,
+                            float(fd), float(fs), hp, L, _dp(out))
+    return (out[0::2] + 1j * out[1::2]).reshape(T, Cc, S)
 
 
 def decode(conv: np.ndarray, H: np.ndarray) -> np.ndarray:
@@ -172,6 +190,14 @@ def pipeline(acq, cubic: bool = True, cf_hercules: bool = False, return_stages: 
     bp = acq.bp
     S, Cc, T = bp.dec_data_dim[0], bp.dec_data_dim[1], bp.dec_data_dim[2]
     R0 = bp.rf_raw_dim[0]
+    if bp.sample_type == 1:  # SAMPLE_BANDPASS: (I, Q) pairs -> demodulation + centred FIR, no Hilbert
+        h = np.array(bp.rf_filter[: bp.filter_length], np.float64) if bp.filter_length else None
+        iq = convert_demod(acq.raw, bp.data_type, np.array(bp.channel_mapping[:Cc], np.int16), R0, T, Cc, S,
+                           bp.center_frequency, bp.sampling_frequency, h)
+        vol, act = das(iq, bp, cubic, cf_hercules)
+        if return_stages:
+            return dict(iq=iq, vol=vol, active=act)
+        return vol
     conv = convert(acq.raw, bp.data_type, np.array(bp.channel_mapping[:Cc], np.int16), R0, T, Cc, S)
     if bp.decode:
         H = hadamard(T, bp.decode)

@@ -120,6 +120,58 @@ void or_convert(const void* raw, int type, const int16_t* map, size_t R0, int T,
         }
 }
 
+/* Bandpass input (data_conversion_kernels.cuh:14-21, :46-64, :66-102;
+ * data_converter.cu:64-131): every two samples of a raw row are an (I, Q)
+ * pair; output sample s of transmit t, channel c is
+ *   v = (I - iQ) exp(-2 pi i s fd / fs),  I, Q = raw[map[c]][2 (t S + s) + 0, 1],
+ * then, with L > 0 taps, the centred FIR
+ *   y[s] = sum_{i < L, 0 <= s + i - L/2 < S} v[s + i - L/2] h[L - 1 - i]  /  (number of such i),
+ * else y = v.  out: interleaved complex [T][C][S]. */
+void or_convert_demod(const void* raw, int type, const int16_t* map, size_t R0, int T, int C, int S, double fd,
+                      double fs, const double* h, int L, double* out)
+{
+#pragma omp parallel for schedule(static)
+    for (int tc = 0; tc < T * C; ++tc) {
+        const int t = tc / C, c = tc % C;
+        const size_t src = (size_t)(uint16_t)map[c] * R0 + 2 * (size_t)t * S;
+        double* v = (double*)malloc(2 * (size_t)S * sizeof(double));
+        for (int k = 0; k < S; ++k) {
+            double I, Q;
+            if (type == TYPE_I16) {
+                I = ((const int16_t*)raw)[src + 2 * (size_t)k];
+                Q = ((const int16_t*)raw)[src + 2 * (size_t)k + 1];
+            } else {
+                I = ((const float*)raw)[src + 2 * (size_t)k];
+                Q = ((const float*)raw)[src + 2 * (size_t)k + 1];
+            }
+            const double ph = 2.0 * M_PI * fmod((double)k * fd / fs, 1.0);
+            const double cr = cos(ph), ci = -sin(ph);
+            v[2 * k] = I * cr + Q * ci;        /* (I - iQ)(cr + i ci) */
+            v[2 * k + 1] = I * ci - Q * cr;
+        }
+        double* y = out + 2 * ((size_t)t * C + c) * S;
+        if (L <= 0) {
+            memcpy(y, v, 2 * (size_t)S * sizeof(double));
+        } else {
+            const int half = L / 2;
+            for (int k = 0; k < S; ++k) {
+                double ar = 0.0, ai = 0.0;
+                int n = 0;
+                for (int i = 0; i < L; ++i) {
+                    const int j = k + i - half;
+                    if (j < 0 || j >= S) continue;
+                    ar += v[2 * j] * h[L - 1 - i];
+                    ai += v[2 * j + 1] * h[L - 1 - i];
+                    ++n;
+                }
+                y[2 * k] = ar / n;
+                y[2 * k + 1] = ai / n;
+            }
+        }
+        free(v);
+    }
+}
+
 /* ------------------------------------------------------------------------ */
 /* DFT / FFT / Hilbert.                                                      */
 /* ------------------------------------------------------------------------ */

@@ -30,6 +30,9 @@ void or_decode(const double* conv, const double* H, int T, size_t cols, double* 
 
 /* ---- convert (data_conversion_kernels.cuh:25-41) ---- */
 void or_convert(const void* raw, int type, const int16_t* map, size_t R0, int T, int C, int S, double* out);
+/* ---- bandpass convert + demodulation + centred FIR (data_conversion_kernels.cuh:14-102) ---- */
+void or_convert_demod(const void* raw, int type, const int16_t* map, size_t R0, int T, int C, int S, double fd,
+                      double fs, const double* h, int L, double* out);
 
 /* ---- FFT / Hilbert (hilbert_handler.cu:55-155, hilbert_kernels.cuh:17-40) ---- */
 void or_dft(const double* re, const double* im, int n, int sign, double* ore, double* oim);
