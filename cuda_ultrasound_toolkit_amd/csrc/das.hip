@@ -164,7 +164,7 @@ struct HercGrid {
 // range [gt_lo, gt_hi).  Writes the final voxels (coherence factor applied
 // when CF), or, when part is not null, the partial sums (acc, incoherent) to
 // part[voxel].  All threads of the workgroup must call it (it synchronises).
-template <bool CUBIC, bool FOCUS, bool CF, bool SINAPO, int IQK>
+template <bool CUBIC, bool FOCUS, bool CF, bool SINAPO, int IQK, bool CLAMP>
 __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float iq_inv, const DasParams& p, int bx,
                                               int by, int bz, int gt_lo, int gt_hi, float2* __restrict__ vol,
                                               float4* __restrict__ part, float2* win, int* wlo_s, int* wmode_s)
@@ -344,8 +344,10 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
                     for (int j = 0; j < HT_Z; ++j) {
                         const float w = tap_weight<SINAPO>(rho, kz[j], apo_c);
                         // the window covers every voxel of the tile with a 2-sample margin,
-                        // so the index needs no clamp beyond the reference's [1, S-2]
-                        const float x = __builtin_amdgcn_fmed3f(fmaf(fsqrt(rho2 + z2[j]), p.spm, base_r[j]), lo_r, hi_r);
+                        // so the index needs no clamp beyond the reference's [1, S-2], and
+                        // none at all in tiles whose every index lies inside it
+                        const float xr = fmaf(fsqrt(rho2 + z2[j]), p.spm, base_r[j]);
+                        const float x = CLAMP ? __builtin_amdgcn_fmed3f(xr, lo_r, hi_r) : xr;
                         const float tt = __builtin_amdgcn_fractf(x);
                         const int li = (int)x;
                         float2 v;
@@ -399,6 +401,48 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
 }
 
 
+// Whether every sample index any (transmit, channel, voxel) of the tile can
+// form lies inside [2, S-3]: then the reference's clamp to [1, S-2] never
+// acts (1-sample margin for rounding) and the linear tap loop drops it (the
+// cubic loop keeps it: without the clamp its schedule needs 24 more VGPRs,
+// one wave per SIMD less).  Bounds: transmit path over the tile
+// box as in hercules_tile, receive path between the tile's depth range and
+// the distance to the farthest element of the aperture.
+template <bool FOCUS>
+__device__ __forceinline__ bool tile_never_clamps(const DasParams& p, int bx, int by, int bz)
+{
+    if (!(p.spm > 0.0f)) return false;
+    const float xa = p.min_x + (float)bx * p.res_x, xb = p.min_x + (float)min(bx + HT_X - 1, p.Nx - 1) * p.res_x;
+    const float ya = p.min_y + (float)by * p.res_y, yb = p.min_y + (float)min(by + HT_Y - 1, p.Ny - 1) * p.res_y;
+    const float za = p.min_z + (float)bz * p.res_z, zb = p.min_z + (float)min(bz + HT_Z - 1, p.Nz - 1) * p.res_z;
+    const float X0 = fminf(xa, xb), X1 = fmaxf(xa, xb), Y0 = fminf(ya, yb), Y1 = fmaxf(ya, yb);
+    const float Z0 = fminf(za, zb), Z1 = fmaxf(za, zb);
+    const MinMax zr = dist_range(0.0f, Z0, Z1);
+    const float fz = p.focal_z;
+    float txd_lo = Z0, txd_hi = Z1;
+    if constexpr (FOCUS) {
+        const MinMax yr = dist_range(0.0f, Y0, Y1);
+        txd_lo = 1e30f;
+        txd_hi = -1e30f;
+        for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 2; ++b) {
+                const float yy = a ? yr.hi : yr.lo, zz = (b ? Z1 : Z0) - fz;
+                const float v = copysignf(sqrtf(yy * yy + zz * zz), zz) + fz;
+                txd_lo = fminf(txd_lo, v);
+                txd_hi = fmaxf(txd_hi, v);
+            }
+    }
+    const int G = p.readi_groups > 1 ? p.readi_groups : 1;
+    const float ex0 = p.xdc_min_x, ex1 = fmaf((float)(p.C - 1), p.pitch_x, p.xdc_min_x);
+    const float ey0 = p.xdc_min_y, ey1 = fmaf((float)(G * p.T - 1), p.pitch_y, p.xdc_min_y);
+    const float dx = fmaxf(fmaxf(fabsf(ex0 - X0), fabsf(ex0 - X1)), fmaxf(fabsf(ex1 - X0), fabsf(ex1 - X1)));
+    const float dy = fmaxf(fmaxf(fabsf(ey0 - Y0), fabsf(ey0 - Y1)), fmaxf(fabsf(ey1 - Y0), fabsf(ey1 - Y1)));
+    const float rmax = sqrtf(dx * dx + dy * dy + zr.hi * zr.hi);
+    const float ilo = fmaf(txd_lo + zr.lo + fz, p.spm, (float)p.delay);
+    const float ihi = fmaf(txd_hi + rmax + fz, p.spm, (float)p.delay);
+    return ilo >= 2.0f && ihi <= (float)p.S - 3.0f;
+}
+
 template <bool CUBIC, bool FOCUS, bool CF, bool SINAPO, int IQK>
 __global__ __launch_bounds__(256) void hercules_kernel(const void* __restrict__ iq, float iq_inv, const DasParams p,
                                                        const HercGrid hg, float2* __restrict__ vol,
@@ -417,8 +461,12 @@ __global__ __launch_bounds__(256) void hercules_kernel(const void* __restrict__ 
     const int gt_lo = split * hg.gt_per_split;
     const int gt_hi = min(G * p.T, gt_lo + hg.gt_per_split);
     float4* pp = hg.splits > 1 ? part + (size_t)split * p.Nx * p.Ny * p.Nz : nullptr;
-    hercules_tile<CUBIC, FOCUS, CF, SINAPO, IQK>(iq, iq_inv, p, bx, by, bz, gt_lo, gt_hi, vol, pp, win, wlo_s,
-                                                 wmode_s);
+    if (!CUBIC && tile_never_clamps<FOCUS>(p, bx, by, bz))
+        hercules_tile<CUBIC, FOCUS, CF, SINAPO, IQK, false>(iq, iq_inv, p, bx, by, bz, gt_lo, gt_hi, vol, pp, win,
+                                                            wlo_s, wmode_s);
+    else
+        hercules_tile<CUBIC, FOCUS, CF, SINAPO, IQK, true>(iq, iq_inv, p, bx, by, bz, gt_lo, gt_hi, vol, pp, win,
+                                                           wlo_s, wmode_s);
 }
 
 // A batch of frames of any sizes in one launch.  Work items (frame, tile) are
@@ -455,8 +503,12 @@ __global__ __launch_bounds__(256) void hercules_batch_kernel(const BatchFrameDev
         const int wz = tile / per_z, wxy = tile - wz * per_z;
         const int bx = (wxy % fr.nxt) * HT_X, by = (wxy / fr.nxt) * HT_Y, bz = (fr.nzt - 1 - wz) * HT_Z;
         const int G = p.readi_groups > 1 ? p.readi_groups : 1;
-        hercules_tile<CUBIC, FOCUS, CF, SINAPO, IQK>(fr.iq, fr.iq_inv, p, bx, by, bz, 0, G * p.T, fr.vol, nullptr,
-                                                     win, wlo_s, wmode_s);
+        if (!CUBIC && tile_never_clamps<FOCUS>(p, bx, by, bz))
+            hercules_tile<CUBIC, FOCUS, CF, SINAPO, IQK, false>(fr.iq, fr.iq_inv, p, bx, by, bz, 0, G * p.T, fr.vol,
+                                                                nullptr, win, wlo_s, wmode_s);
+        else
+            hercules_tile<CUBIC, FOCUS, CF, SINAPO, IQK, true>(fr.iq, fr.iq_inv, p, bx, by, bz, 0, G * p.T, fr.vol,
+                                                               nullptr, win, wlo_s, wmode_s);
         __syncthreads();
         if (threadIdx.x == 0) {
             __threadfence();
