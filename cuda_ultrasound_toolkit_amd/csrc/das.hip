@@ -232,6 +232,7 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
 
     // Aperture radius (Hann): rho < 2*A*z/fn.  Unbounded when fn <= 0 or z <= 0.
     const bool aperture_bounded = !SINAPO && p.fn_rx > 0.0f && Z0 > 0.0f && p.pitch_x > 0.0f;
+    const bool unbounded = !(p.fn_rx > 0.0f);  // no F-number: every lane is inside the aperture
     const float rmax_ap = aperture_bounded ? 2.0f * (HANN_A_THR * 1.001f) * Z1 / p.fn_rx + 1e-6f : 0.0f;
 
     float2 acc[HT_Z];
@@ -305,8 +306,7 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
                 __syncthreads();
                 // per channel: lateral distance, apodization inputs, and whether any
                 // lane of the wave can be inside the aperture (uniform skip)
-                auto chan = [&](int cc, float& rho2, float& rho, float& apo_c) -> bool {
-                    const float ex = fmaf((float)(cb + cc), p.pitch_x, p.xdc_min_x);
+                auto chan = [&](float ex, float& rho2, float& rho, float& apo_c) -> bool {
                     const float dx = ex - vx;
                     rho2 = fmaf(dx, dx, dy2);
                     rho = 0.0f;
@@ -317,16 +317,25 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
                     } else {
                         rho = fsqrt(rho2);
                         // the deepest voxel of the column has the widest aperture
-                        return __any(rho * kz[HT_Z - 1] < HANN_A_THR * 1.001f || p.fn_rx <= 0.0f);
+                        return unbounded || __builtin_amdgcn_ballot_w64(rho * kz[HT_Z - 1] < HANN_A_THR * 1.001f) != 0;
                     }
                 };
                 // Sample indices of this block of channels are taken relative to
                 // the smallest window start of the block (an integer, so fractions
                 // are unchanged and indices stay small and non-negative); the
                 // per-channel window offset then moves into the uniform LDS base.
-                int org = 0x7fffffff;
-                for (int cc = 0; cc < nC; ++cc)
-                    if (wmode_s[cc] == 0) org = min(org, wlo_s[cc]);
+                // Lane cc of every wave holds channel cc's element x and LDS byte
+                // base (-1: window in HBM); the channel loops read them with
+                // readlane, so their bookkeeping stays in scalar registers.
+                const int lane = tid & 63;
+                const int my_mode = lane < nC ? wmode_s[lane] : 1;
+                const int my_wlo = lane < nC ? wlo_s[lane] : 0x7fffffff;
+                int org = my_mode == 0 ? my_wlo : 0x7fffffff;
+#pragma unroll
+                for (int off = 8; off > 0; off >>= 1) org = min(org, __shfl_xor(org, off, 16));
+                org = __builtin_amdgcn_readfirstlane(org);
+                const int my_base = my_mode == 0 ? (lane * HW - (my_wlo - org)) * (int)sizeof(float2) : -1;
+                const float my_ex = fmaf((float)(cb + lane), p.pitch_x, p.xdc_min_x);
                 const float forg = (float)org;
                 const float lo_r = lo_clamp - forg, hi_r = hi_clamp - forg;
                 float base_r[HT_Z];
@@ -335,11 +344,13 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
                 // channels whose window sits in LDS; a single path keeps the
                 // accumulators in place across iterations
                 for (int cc = 0; cc < nC; ++cc) {
-                    if (wmode_s[cc] != 0) continue;
+                    const int wb = __builtin_amdgcn_readlane(my_base, cc);
+                    if (wb < 0) continue;
                     float rho2, rho, apo_c;
-                    if (!chan(cc, rho2, rho, apo_c)) continue;
+                    if (!chan(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(my_ex), cc)), rho2, rho, apo_c))
+                        continue;
                     // window row of channel cc, indexed by block-relative sample index
-                    const float2* wrow = win + __builtin_amdgcn_readfirstlane(cc * HW - (wlo_s[cc] - org));
+                    const float2* wrow = reinterpret_cast<const float2*>(reinterpret_cast<const char*>(win) + wb);
 #pragma unroll
                     for (int j = 0; j < HT_Z; ++j) {
                         const float w = tap_weight<SINAPO>(rho, kz[j], apo_c);
@@ -360,9 +371,10 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
                 }
                 // channels whose window did not fit: direct reads
                 for (int cc = 0; cc < nC; ++cc) {
-                    if (wmode_s[cc] == 0) continue;
+                    if (__builtin_amdgcn_readlane(my_base, cc) >= 0) continue;
                     float rho2, rho, apo_c;
-                    if (!chan(cc, rho2, rho, apo_c)) continue;
+                    if (!chan(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(my_ex), cc)), rho2, rho, apo_c))
+                        continue;
                     const size_t grow = toff + (size_t)(cb + cc) * rowlen;
 #pragma unroll
                     for (int j = 0; j < HT_Z; ++j) {
@@ -443,8 +455,11 @@ __device__ __forceinline__ bool tile_never_clamps(const DasParams& p, int bx, in
     return ilo >= 2.0f && ihi <= (float)p.S - 3.0f;
 }
 
+// Four workgroups per CU (one wave per SIMD each, four waves per SIMD): the
+// register budget is then 128 VGPRs, which the linear and cubic loops meet
+// without spilling (the coherence-factor variants spill a few bytes).
 template <bool CUBIC, bool FOCUS, bool CF, bool SINAPO, int IQK>
-__global__ __launch_bounds__(256) void hercules_kernel(const void* __restrict__ iq, float iq_inv, const DasParams p,
+__global__ __launch_bounds__(256, 4) void hercules_kernel(const void* __restrict__ iq, float iq_inv, const DasParams p,
                                                        const HercGrid hg, float2* __restrict__ vol,
                                                        float4* __restrict__ part)
 {
