@@ -190,6 +190,9 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
     const MinMax zr = dist_range(0.0f, Z0, Z1);  // |z| range
 
     const float fz = p.focal_z;
+    // the tile's 8 depths are the same for every lane: z-only values go to
+    // scalar registers (readfirstlane), which frees 16 VGPRs
+    auto uni = [](float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); };
     // per-voxel constants
     float z2[HT_Z], kz[HT_Z], base[HT_Z];
     double base_d[HT_Z];  // the same in FP64, anchored per channel block below
@@ -199,8 +202,8 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
         const int iz = bz + j;
         zvalid[j] = iz < p.Nz;
         const float z = p.min_z + (float)min(iz, p.Nz - 1) * p.res_z;
-        z2[j] = z * z;
-        kz[j] = p.fn_rx / z * 0.5f;
+        z2[j] = uni(z * z);
+        kz[j] = uni(p.fn_rx / z * 0.5f);
         float txd;
         if constexpr (FOCUS) {
             const float ty = vy, tzz = z - fz;
@@ -491,7 +494,7 @@ __global__ __launch_bounds__(256, 4) void hercules_kernel(const void* __restrict
 // (no split).  When a frame's last tile is done, its completion time
 // (wall clock) is recorded.
 template <bool CUBIC, bool FOCUS, bool CF, bool SINAPO, int IQK>
-__global__ __launch_bounds__(256) void hercules_batch_kernel(const BatchFrameDev* __restrict__ frames,
+__global__ __launch_bounds__(256, 4) void hercules_batch_kernel(const BatchFrameDev* __restrict__ frames,
                                                              const int2* __restrict__ items, int n_items,
                                                              int* __restrict__ next, int* __restrict__ done,
                                                              long long* __restrict__ finish)
