@@ -31,9 +31,24 @@ namespace cut {
 
 constexpr int HT_X = 16, HT_Y = 16, HT_Z = 8;  // voxel tile
 constexpr int HCB = 16;                         // channels staged per pass
-constexpr int HW = 256;                         // window length (complex samples)
+constexpr int HW = 256;                         // longest window staged in LDS (samples)
+// LDS window pool of a workgroup: 32 KiB, shared by the channels of a pass at
+// offsets given by a prefix sum of their window lengths.  Linear
+// interpolation stores one float4 (p[i], p[i+1] - p[i]) per sample, cubic one
+// float2 p[i].
+constexpr int HPOOL_BYTES = 32768;
 // apo > 0.1  <=>  fn*rho/(2z) < acos(sqrt(0.1))/pi for the Hann window.
 constexpr float HANN_A_THR = 0.39758361765043326f;
+constexpr float HANN_S_THR = HANN_A_THR * HANN_A_THR;
+// cos^2(pi a) as a polynomial in s = a^2 on [0, HANN_S_THR] (where the
+// threshold keeps the weight): near-minimax degree 4 with the constant term
+// fixed at 1, max error 2.6e-6 in FP32 Horner form; replaces v_cos_f32, which
+// issues at a quarter of the VALU rate.
+constexpr float HANN_P1 = -9.86909866f, HANN_P2 = 32.4382935f, HANN_P3 = -42.1193924f, HANN_P4 = 25.4091187f;
+__device__ __forceinline__ float hann_poly(float s)
+{
+    return fmaf(fmaf(fmaf(fmaf(HANN_P4, s, HANN_P3), s, HANN_P2), s, HANN_P1), s, 1.0f);
+}
 
 __device__ __forceinline__ float cos_2pi(float x) { return __builtin_amdgcn_cosf(x); }
 // v_sqrt_f32 (1 ulp): the IEEE-exact sqrtf expansion costs ~15 VALU ops per call.
@@ -120,16 +135,17 @@ __device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v,
 // Weight of one tap: apodization when it passes the reference's threshold
 // (apo > 0.1), else 0.  (Lanes of voxels outside the grid compute in-range
 // values that are never stored.)
+// rho2 = squared lateral distance, kz2j = (fn / (2 z))^2 of the voxel's depth.
 template <bool SINAPO>
-__device__ __forceinline__ float tap_weight(float rho, float kzj, float apo_c)
+__device__ __forceinline__ float tap_weight(float rho2, float kz2j, float apo_c)
 {
     if constexpr (SINAPO) {
         return apo_c > 0.1f ? apo_c : 0.0f;
     } else {
         // cos^2(pi a) > 0.1 <=> a < acos(sqrt(0.1))/pi on [0, 0.5]; for those a
         // the reference's min(a, 0.5) clamp is inactive
-        const float a = rho * kzj;
-        return fabsf(a) < HANN_A_THR ? fmaf(0.5f, cos_2pi(a), 0.5f) : 0.0f;
+        const float s = rho2 * kz2j;
+        return s < HANN_S_THR ? hann_poly(s) : 0.0f;
     }
 }
 
@@ -167,8 +183,12 @@ struct HercGrid {
 template <bool CUBIC, bool FOCUS, bool CF, bool SINAPO, int IQK, bool CLAMP>
 __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float iq_inv, const DasParams& p, int bx,
                                               int by, int bz, int gt_lo, int gt_hi, float2* __restrict__ vol,
-                                              float4* __restrict__ part, float2* win, int* wlo_s, int* wmode_s)
+                                              float4* __restrict__ part, float4* win, int* wlo_s, int* wmode_s,
+                                              int* woff_s)
 {
+    // window entry size (bytes) and pool capacity (entries)
+    constexpr int WIN_ESZ = CUBIC ? (int)sizeof(float2) : (int)sizeof(float4);
+    constexpr int HPOOL = HPOOL_BYTES / WIN_ESZ;
     const int tid = threadIdx.x;
     // each wave owns an 8 x 8 patch of columns: a compact footprint keeps the
     // wave's aperture test coherent, so more channels are skipped whole
@@ -194,7 +214,7 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
     // scalar registers (readfirstlane), which frees 16 VGPRs
     auto uni = [](float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); };
     // per-voxel constants
-    float z2[HT_Z], kz[HT_Z], base[HT_Z];
+    float z2[HT_Z], kz2[HT_Z], base[HT_Z];
     double base_d[HT_Z];  // the same in FP64, anchored per channel block below
     bool zvalid[HT_Z];
 #pragma unroll
@@ -203,7 +223,8 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
         zvalid[j] = iz < p.Nz;
         const float z = p.min_z + (float)min(iz, p.Nz - 1) * p.res_z;
         z2[j] = uni(z * z);
-        kz[j] = uni(p.fn_rx / z * 0.5f);
+        const float kz = p.fn_rx / z * 0.5f;
+        kz2[j] = uni(kz * kz);
         float txd;
         if constexpr (FOCUS) {
             const float ty = vy, tzz = z - fz;
@@ -283,9 +304,14 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
                 const int nC = min(HCB, c_hi - cb + 1);
                 __syncthreads();
                 {
+                    // Lane k of every 16-lane group bounds the window of channel
+                    // cb + k (the sample span any voxel of the tile can read); an
+                    // exclusive prefix sum over the group places the windows in
+                    // the pool.  Group wi then stages channel cb + wi.
                     const int wi = tid >> 4, part = tid & 15;
-                    if (wi < nC) {
-                        const int c = cb + wi;
+                    int wlo = 0, len = 0;
+                    if (part < nC) {
+                        const int c = cb + part;
                         const float ex = fmaf((float)c, p.pitch_x, p.xdc_min_x);
                         const MinMax dxr = dist_range(ex, X0, X1);
                         const float rmin = sqrtf(dxr.lo * dxr.lo + dyr.lo * dyr.lo + zr.lo * zr.lo);
@@ -295,49 +321,75 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
                         if (p.spm < 0.0f) { const float tt = ilo; ilo = ihi; ihi = tt; }
                         ilo = fminf(fmaxf(ilo, lo_clamp), hi_clamp);
                         ihi = fminf(fmaxf(ihi, lo_clamp), hi_clamp);
-                        const int wlo = max(0, (int)floorf(ilo) - 2);
+                        wlo = max(0, (int)floorf(ilo) - 2);
                         const int whi = min(p.S - 1, (int)floorf(ihi) + 3);
-                        const int len = whi - wlo + 1;
-                        const int mode = (len <= HW && len > 0) ? 0 : 1;
-                        if (part == 0) { wlo_s[wi] = wlo; wmode_s[wi] = mode; }
-                        if (mode == 0) {
-                            const size_t row = toff + (size_t)c * rowlen + wlo;
-                            for (int q = part; q < len; q += 16) win[wi * HW + q] = iq_at<IQK>(iq, row + q, iq_inv);
+                        len = whi - wlo + 1;
+                    }
+                    const bool fits = part < nC && len > 0 && len <= HW;
+                    const int need = fits ? len : 0;
+                    int scan = need;
+#pragma unroll
+                    for (int off = 1; off < 16; off <<= 1) {
+                        const int v = __shfl_up(scan, off, 16);
+                        if (part >= off) scan += v;
+                    }
+                    const int mode = (fits && scan <= HPOOL) ? 0 : 1;
+                    const int c_wlo = __shfl(wlo, wi, 16), c_len = __shfl(len, wi, 16);
+                    const int c_off = __shfl(scan - need, wi, 16), c_mode = __shfl(mode, wi, 16);
+                    if (wi < nC) {
+                        if (part == 0) { wlo_s[wi] = c_wlo; wmode_s[wi] = c_mode; woff_s[wi] = c_off; }
+                        if (c_mode == 0) {
+                            const size_t row = toff + (size_t)(cb + wi) * rowlen;
+                            if constexpr (CUBIC) {
+                                float2* w2 = reinterpret_cast<float2*>(win) + c_off;
+                                for (int q = part; q < c_len; q += 16) w2[q] = iq_at<IQK>(iq, row + c_wlo + q, iq_inv);
+                            } else {
+                                // (p[i], p[i+1] - p[i]); the entry of sample S-1 is never read
+                                float4* w4 = win + c_off;
+                                for (int q = part; q < c_len; q += 16) {
+                                    const int i = c_wlo + q;
+                                    const float2 a = iq_at<IQK>(iq, row + i, iq_inv);
+                                    const float2 b = iq_at<IQK>(iq, row + min(i + 1, p.S - 1), iq_inv);
+                                    w4[q] = make_float4(a.x, a.y, b.x - a.x, b.y - a.y);
+                                }
+                            }
                         }
                     }
                 }
                 __syncthreads();
                 // per channel: lateral distance, apodization inputs, and whether any
                 // lane of the wave can be inside the aperture (uniform skip)
-                auto chan = [&](float ex, float& rho2, float& rho, float& apo_c) -> bool {
+                auto chan = [&](float ex, float& rho2, float& apo_c) -> bool {
                     const float dx = ex - vx;
                     rho2 = fmaf(dx, dx, dy2);
-                    rho = 0.0f;
                     apo_c = 0.0f;
                     if constexpr (SINAPO) {
                         apo_c = sin_apo_to(ex, p.xdc_max_x, p.to_power) * apo_y;
                         return apo_c > 0.1f;  // uniform: depends on the element only
                     } else {
-                        rho = fsqrt(rho2);
                         // the deepest voxel of the column has the widest aperture
-                        return unbounded || __builtin_amdgcn_ballot_w64(rho * kz[HT_Z - 1] < HANN_A_THR * 1.001f) != 0;
+                        constexpr float thr = (HANN_A_THR * 1.001f) * (HANN_A_THR * 1.001f);
+                        return unbounded || __builtin_amdgcn_ballot_w64(rho2 * kz2[HT_Z - 1] < thr) != 0;
                     }
                 };
                 // Sample indices of this block of channels are taken relative to
                 // the smallest window start of the block (an integer, so fractions
                 // are unchanged and indices stay small and non-negative); the
-                // per-channel window offset then moves into the uniform LDS base.
-                // Lane cc of every wave holds channel cc's element x and LDS byte
-                // base (-1: window in HBM); the channel loops read them with
-                // readlane, so their bookkeeping stays in scalar registers.
+                // per-channel window offset then moves into the uniform LDS base
+                // (a byte offset that may be negative: every index read is at
+                // least the window's start).  Lane cc of every wave holds channel
+                // cc's element x and LDS base, bit cc of lds_mask whether its
+                // window is in LDS; the channel loops read them with readlane, so
+                // their bookkeeping stays in scalar registers.
                 const int lane = tid & 63;
-                const int my_mode = lane < nC ? wmode_s[lane] : 1;
-                const int my_wlo = lane < nC ? wlo_s[lane] : 0x7fffffff;
-                int org = my_mode == 0 ? my_wlo : 0x7fffffff;
+                const bool my_lds = lane < nC && wmode_s[lane] == 0;
+                const int my_wlo = my_lds ? wlo_s[lane] : 0x7fffffff;
+                const unsigned long long lds_mask = __builtin_amdgcn_ballot_w64(my_lds);
+                int org = my_wlo;
 #pragma unroll
                 for (int off = 8; off > 0; off >>= 1) org = min(org, __shfl_xor(org, off, 16));
                 org = __builtin_amdgcn_readfirstlane(org);
-                const int my_base = my_mode == 0 ? (lane * HW - (my_wlo - org)) * (int)sizeof(float2) : -1;
+                const int my_base = my_lds ? (woff_s[lane] - (my_wlo - org)) * (int)WIN_ESZ : 0;
                 const float my_ex = fmaf((float)(cb + lane), p.pitch_x, p.xdc_min_x);
                 const float forg = (float)org;
                 const float lo_r = lo_clamp - forg, hi_r = hi_clamp - forg;
@@ -347,26 +399,30 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
                 // channels whose window sits in LDS; a single path keeps the
                 // accumulators in place across iterations
                 for (int cc = 0; cc < nC; ++cc) {
+                    if (!((lds_mask >> cc) & 1ull)) continue;
                     const int wb = __builtin_amdgcn_readlane(my_base, cc);
-                    if (wb < 0) continue;
-                    float rho2, rho, apo_c;
-                    if (!chan(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(my_ex), cc)), rho2, rho, apo_c))
+                    float rho2, apo_c;
+                    if (!chan(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(my_ex), cc)), rho2, apo_c))
                         continue;
-                    // window row of channel cc, indexed by block-relative sample index
-                    const float2* wrow = reinterpret_cast<const float2*>(reinterpret_cast<const char*>(win) + wb);
+                    const char* wbytes = reinterpret_cast<const char*>(win);
 #pragma unroll
                     for (int j = 0; j < HT_Z; ++j) {
-                        const float w = tap_weight<SINAPO>(rho, kz[j], apo_c);
+                        const float w = tap_weight<SINAPO>(rho2, kz2[j], apo_c);
                         // the window covers every voxel of the tile with a 2-sample margin,
                         // so the index needs no clamp beyond the reference's [1, S-2], and
                         // none at all in tiles whose every index lies inside it
                         const float xr = fmaf(fsqrt(rho2 + z2[j]), p.spm, base_r[j]);
                         const float x = CLAMP ? __builtin_amdgcn_fmed3f(xr, lo_r, hi_r) : xr;
                         const float tt = __builtin_amdgcn_fractf(x);
-                        const int li = (int)x;
+                        const int addr = wb + (int)x * (int)WIN_ESZ;  // window entry of block-relative index
                         float2 v;
-                        if constexpr (CUBIC) v = cubic2(wrow[li - 1], wrow[li], wrow[li + 1], wrow[li + 2], tt);
-                        else v = lerp2(wrow[li], wrow[li + 1], tt);
+                        if constexpr (CUBIC) {
+                            const float2* e = reinterpret_cast<const float2*>(wbytes + addr);
+                            v = cubic2(e[-1], e[0], e[1], e[2], tt);
+                        } else {
+                            const float4 e = *reinterpret_cast<const float4*>(wbytes + addr);
+                            v = make_float2(fmaf(tt, e.z, e.x), fmaf(tt, e.w, e.y));
+                        }
                         acc[j].x = fmaf(w, v.x, acc[j].x);
                         acc[j].y = fmaf(w, v.y, acc[j].y);
                         if constexpr (CF) inc[j] = (w != 0.0f) ? fmaf(v.x, v.x, fmaf(v.y, v.y, inc[j])) : inc[j];
@@ -374,14 +430,14 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
                 }
                 // channels whose window did not fit: direct reads
                 for (int cc = 0; cc < nC; ++cc) {
-                    if (__builtin_amdgcn_readlane(my_base, cc) >= 0) continue;
-                    float rho2, rho, apo_c;
-                    if (!chan(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(my_ex), cc)), rho2, rho, apo_c))
+                    if ((lds_mask >> cc) & 1ull) continue;
+                    float rho2, apo_c;
+                    if (!chan(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(my_ex), cc)), rho2, apo_c))
                         continue;
                     const size_t grow = toff + (size_t)(cb + cc) * rowlen;
 #pragma unroll
                     for (int j = 0; j < HT_Z; ++j) {
-                        const float w = tap_weight<SINAPO>(rho, kz[j], apo_c);
+                        const float w = tap_weight<SINAPO>(rho2, kz2[j], apo_c);
                         if (w == 0.0f) continue;
                         const float x = tap_index(rho2 + z2[j], p.spm, base[j], hi_clamp);
                         const float2 v = interp_global<CUBIC, IQK>(iq, grow, x, p.S, iq_inv);
@@ -466,9 +522,10 @@ __global__ __launch_bounds__(256, 4) void hercules_kernel(const void* __restrict
                                                        const HercGrid hg, float2* __restrict__ vol,
                                                        float4* __restrict__ part)
 {
-    __shared__ float2 win[HCB * HW];
+    __shared__ float4 win[HPOOL_BYTES / sizeof(float4)];
     __shared__ int wlo_s[HCB];
     __shared__ int wmode_s[HCB];
+    __shared__ int woff_s[HCB];
     const int per_z = hg.nxt * hg.nyt * hg.splits;
     const int wz = blockIdx.x / per_z;
     const int wr = blockIdx.x - wz * per_z;
@@ -481,10 +538,10 @@ __global__ __launch_bounds__(256, 4) void hercules_kernel(const void* __restrict
     float4* pp = hg.splits > 1 ? part + (size_t)split * p.Nx * p.Ny * p.Nz : nullptr;
     if (!CUBIC && tile_never_clamps<FOCUS>(p, bx, by, bz))
         hercules_tile<CUBIC, FOCUS, CF, SINAPO, IQK, false>(iq, iq_inv, p, bx, by, bz, gt_lo, gt_hi, vol, pp, win,
-                                                            wlo_s, wmode_s);
+                                                            wlo_s, wmode_s, woff_s);
     else
         hercules_tile<CUBIC, FOCUS, CF, SINAPO, IQK, true>(iq, iq_inv, p, bx, by, bz, gt_lo, gt_hi, vol, pp, win,
-                                                           wlo_s, wmode_s);
+                                                           wlo_s, wmode_s, woff_s);
 }
 
 // A batch of frames of any sizes in one launch.  Work items (frame, tile) are
@@ -499,9 +556,10 @@ __global__ __launch_bounds__(256, 4) void hercules_batch_kernel(const BatchFrame
                                                              int* __restrict__ next, int* __restrict__ done,
                                                              long long* __restrict__ finish)
 {
-    __shared__ float2 win[HCB * HW];
+    __shared__ float4 win[HPOOL_BYTES / sizeof(float4)];
     __shared__ int wlo_s[HCB];
     __shared__ int wmode_s[HCB];
+    __shared__ int woff_s[HCB];
     __shared__ int item_s;
     const bool queue = gridDim.x < (unsigned)n_items;
     int it = blockIdx.x;
@@ -523,10 +581,10 @@ __global__ __launch_bounds__(256, 4) void hercules_batch_kernel(const BatchFrame
         const int G = p.readi_groups > 1 ? p.readi_groups : 1;
         if (!CUBIC && tile_never_clamps<FOCUS>(p, bx, by, bz))
             hercules_tile<CUBIC, FOCUS, CF, SINAPO, IQK, false>(fr.iq, fr.iq_inv, p, bx, by, bz, 0, G * p.T, fr.vol,
-                                                                nullptr, win, wlo_s, wmode_s);
+                                                                nullptr, win, wlo_s, wmode_s, woff_s);
         else
             hercules_tile<CUBIC, FOCUS, CF, SINAPO, IQK, true>(fr.iq, fr.iq_inv, p, bx, by, bz, 0, G * p.T, fr.vol,
-                                                               nullptr, win, wlo_s, wmode_s);
+                                                               nullptr, win, wlo_s, wmode_s, woff_s);
         __syncthreads();
         if (threadIdx.x == 0) {
             __threadfence();
