@@ -22,9 +22,15 @@ static bool g_user_stream_set = false;
 hipStream_t stream()
 {
     if (g_user_stream_set) return g_user_stream;
+    // A blocking stream: it is ordered with the legacy default stream, as the
+    // reference's calls are, so device buffers a caller fills or reads there
+    // (torch's default stream, for one) need no extra synchronisation around
+    // the device-pointer entry points.  A non-blocking stream raced such
+    // callers: a fill of the output issued before the call could land after
+    // the library's kernels.
     static hipStream_t s = [] {
         hipStream_t h = nullptr;
-        if (hipStreamCreateWithFlags(&h, hipStreamNonBlocking) != hipSuccess) h = nullptr;
+        if (hipStreamCreateWithFlags(&h, hipStreamDefault) != hipSuccess) h = nullptr;
         return h;
     }();
     return s;

@@ -3,7 +3,10 @@
  *
  * Plain C, plain pointers and sizes.  "d_" pointers are device (HBM)
  * pointers; these entry points enqueue on the library stream and return
- * without waiting unless stated, so a caller can time or overlap them.  They
+ * without waiting unless stated, so a caller can time or overlap them.  The
+ * library's own stream is a blocking stream (ordered with the legacy default
+ * stream); work on other non-blocking streams must be ordered by the caller,
+ * or the caller's stream passed with cut_set_stream.  They
  * exist for callers whose data is already resident in HBM (benchmarks,
  * pipelines that produce RF on the GPU), for per-stage testing, and to select
  * the build's precision/interpolation variants.  All return false and print a
