@@ -30,12 +30,14 @@
 namespace cut {
 
 constexpr int HT_X = 16, HT_Y = 16, HT_Z = 8;  // voxel tile
-constexpr int HCB = 16;                         // channels staged per pass
+constexpr int HCB = 16;                         // most channels staged per pass
 constexpr int HW = 256;                         // longest window staged in LDS (samples)
 // LDS window pool of a workgroup: 32 KiB, shared by the channels of a pass at
-// offsets given by a prefix sum of their window lengths.  Linear
-// interpolation stores one float4 (p[i], p[i+1] - p[i]) per sample, cubic one
-// float2 p[i].
+// offsets given by a prefix sum of their window lengths.  Each sample i of a
+// window is stored as the coefficients of the interpolant on [i, i+1]:
+// linear, one float4 (p[i], p[i+1] - p[i]); cubic, two float4 holding the
+// Catmull-Rom polynomial c0 + c1 t + c2 t^2 + c3 t^3 (16 channels per pass for
+// linear, 8 for cubic).
 constexpr int HPOOL_BYTES = 32768;
 // apo > 0.1  <=>  fn*rho/(2z) < acos(sqrt(0.1))/pi for the Hann window.
 constexpr float HANN_A_THR = 0.39758361765043326f;
@@ -186,9 +188,12 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
                                               float4* __restrict__ part, float4* win, int* wlo_s, int* wmode_s,
                                               int* woff_s)
 {
-    // window entry size (bytes) and pool capacity (entries)
-    constexpr int WIN_ESZ = CUBIC ? (int)sizeof(float2) : (int)sizeof(float4);
+    // window entry size (bytes), pool capacity (entries), channels per pass
+    // and staging threads per channel
+    constexpr int WIN_ESZ = CUBIC ? 2 * (int)sizeof(float4) : (int)sizeof(float4);
     constexpr int HPOOL = HPOOL_BYTES / WIN_ESZ;
+    constexpr int NCB = CUBIC ? HCB / 2 : HCB;
+    constexpr int TPC = 256 / NCB;
     const int tid = threadIdx.x;
     // each wave owns an 8 x 8 patch of columns: a compact footprint keeps the
     // wave's aperture test coherent, so more channels are skipped whole
@@ -300,18 +305,18 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
             float apo_y = 1.0f;
             if constexpr (SINAPO) apo_y = sin_apo_to(ey, p.xdc_max_y, p.to_power);
 
-            for (int cb = c_lo; cb <= c_hi; cb += HCB) {
-                const int nC = min(HCB, c_hi - cb + 1);
+            for (int cb = c_lo; cb <= c_hi; cb += NCB) {
+                const int nC = min(NCB, c_hi - cb + 1);
                 __syncthreads();
                 {
-                    // Lane k of every 16-lane group bounds the window of channel
+                    // Lane k of every NCB-lane group bounds the window of channel
                     // cb + k (the sample span any voxel of the tile can read); an
                     // exclusive prefix sum over the group places the windows in
-                    // the pool.  Group wi then stages channel cb + wi.
-                    const int wi = tid >> 4, part = tid & 15;
+                    // the pool.  Threads [wi*TPC, (wi+1)*TPC) then stage channel cb + wi.
+                    const int kb = tid & (NCB - 1), wi = tid / TPC, part = tid % TPC;
                     int wlo = 0, len = 0;
-                    if (part < nC) {
-                        const int c = cb + part;
+                    if (kb < nC) {
+                        const int c = cb + kb;
                         const float ex = fmaf((float)c, p.pitch_x, p.xdc_min_x);
                         const MinMax dxr = dist_range(ex, X0, X1);
                         const float rmin = sqrtf(dxr.lo * dxr.lo + dyr.lo * dyr.lo + zr.lo * zr.lo);
@@ -325,28 +330,46 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
                         const int whi = min(p.S - 1, (int)floorf(ihi) + 3);
                         len = whi - wlo + 1;
                     }
-                    const bool fits = part < nC && len > 0 && len <= HW;
+                    const bool fits = kb < nC && len > 0 && len <= HW;
                     const int need = fits ? len : 0;
                     int scan = need;
 #pragma unroll
-                    for (int off = 1; off < 16; off <<= 1) {
-                        const int v = __shfl_up(scan, off, 16);
-                        if (part >= off) scan += v;
+                    for (int off = 1; off < NCB; off <<= 1) {
+                        const int v = __shfl_up(scan, off, NCB);
+                        if (kb >= off) scan += v;
                     }
                     const int mode = (fits && scan <= HPOOL) ? 0 : 1;
-                    const int c_wlo = __shfl(wlo, wi, 16), c_len = __shfl(len, wi, 16);
-                    const int c_off = __shfl(scan - need, wi, 16), c_mode = __shfl(mode, wi, 16);
+                    const int c_wlo = __shfl(wlo, wi, NCB), c_len = __shfl(len, wi, NCB);
+                    const int c_off = __shfl(scan - need, wi, NCB), c_mode = __shfl(mode, wi, NCB);
                     if (wi < nC) {
                         if (part == 0) { wlo_s[wi] = c_wlo; wmode_s[wi] = c_mode; woff_s[wi] = c_off; }
                         if (c_mode == 0) {
                             const size_t row = toff + (size_t)(cb + wi) * rowlen;
                             if constexpr (CUBIC) {
-                                float2* w2 = reinterpret_cast<float2*>(win) + c_off;
-                                for (int q = part; q < c_len; q += 16) w2[q] = iq_at<IQK>(iq, row + c_wlo + q, iq_inv);
+                                // Catmull-Rom on [i, i+1] (tangents (p[i+1] - p[i-1]) / 2):
+                                // c0 = p0, c1 = (p1 - pm1)/2, c2 = pm1 - 2.5 p0 + 2 p1 - p2/2,
+                                // c3 = (p2 - pm1)/2 + 1.5 (p0 - p1); neighbours outside the row
+                                // are clamped into it: the entries that need them are read
+                                // only at t = 0 (sample S-2 at the clamp) or never
+                                float4* w4 = win + 2 * c_off;
+                                for (int q = part; q < c_len; q += TPC) {
+                                    const int i = c_wlo + q;
+                                    const float2 pm = iq_at<IQK>(iq, row + max(i - 1, 0), iq_inv);
+                                    const float2 p0 = iq_at<IQK>(iq, row + i, iq_inv);
+                                    const float2 p1 = iq_at<IQK>(iq, row + min(i + 1, p.S - 1), iq_inv);
+                                    const float2 p2 = iq_at<IQK>(iq, row + min(i + 2, p.S - 1), iq_inv);
+                                    const float c1x = 0.5f * (p1.x - pm.x), c1y = 0.5f * (p1.y - pm.y);
+                                    const float c2x = fmaf(-0.5f, p2.x, fmaf(2.0f, p1.x, fmaf(-2.5f, p0.x, pm.x)));
+                                    const float c2y = fmaf(-0.5f, p2.y, fmaf(2.0f, p1.y, fmaf(-2.5f, p0.y, pm.y)));
+                                    const float c3x = fmaf(1.5f, p0.x - p1.x, 0.5f * (p2.x - pm.x));
+                                    const float c3y = fmaf(1.5f, p0.y - p1.y, 0.5f * (p2.y - pm.y));
+                                    w4[2 * q] = make_float4(p0.x, p0.y, c1x, c1y);
+                                    w4[2 * q + 1] = make_float4(c2x, c2y, c3x, c3y);
+                                }
                             } else {
                                 // (p[i], p[i+1] - p[i]); the entry of sample S-1 is never read
                                 float4* w4 = win + c_off;
-                                for (int q = part; q < c_len; q += 16) {
+                                for (int q = part; q < c_len; q += TPC) {
                                     const int i = c_wlo + q;
                                     const float2 a = iq_at<IQK>(iq, row + i, iq_inv);
                                     const float2 b = iq_at<IQK>(iq, row + min(i + 1, p.S - 1), iq_inv);
@@ -417,8 +440,10 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
                         const int addr = wb + (int)x * (int)WIN_ESZ;  // window entry of block-relative index
                         float2 v;
                         if constexpr (CUBIC) {
-                            const float2* e = reinterpret_cast<const float2*>(wbytes + addr);
-                            v = cubic2(e[-1], e[0], e[1], e[2], tt);
+                            const float4 e0 = *reinterpret_cast<const float4*>(wbytes + addr);
+                            const float4 e1 = *reinterpret_cast<const float4*>(wbytes + addr + 16);
+                            v = make_float2(fmaf(fmaf(fmaf(e1.z, tt, e1.x), tt, e0.z), tt, e0.x),
+                                            fmaf(fmaf(fmaf(e1.w, tt, e1.y), tt, e0.w), tt, e0.y));
                         } else {
                             const float4 e = *reinterpret_cast<const float4*>(wbytes + addr);
                             v = make_float2(fmaf(tt, e.z, e.x), fmaf(tt, e.w, e.y));
@@ -474,9 +499,8 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
 
 // Whether every sample index any (transmit, channel, voxel) of the tile can
 // form lies inside [2, S-3]: then the reference's clamp to [1, S-2] never
-// acts (1-sample margin for rounding) and the linear tap loop drops it (the
-// cubic loop keeps it: without the clamp its schedule needs 24 more VGPRs,
-// one wave per SIMD less).  Bounds: transmit path over the tile
+// acts (1-sample margin for rounding) and the tap loop drops it.  Bounds:
+// transmit path over the tile
 // box as in hercules_tile, receive path between the tile's depth range and
 // the distance to the farthest element of the aperture.
 template <bool FOCUS>
@@ -536,7 +560,7 @@ __global__ __launch_bounds__(256, 4) void hercules_kernel(const void* __restrict
     const int gt_lo = split * hg.gt_per_split;
     const int gt_hi = min(G * p.T, gt_lo + hg.gt_per_split);
     float4* pp = hg.splits > 1 ? part + (size_t)split * p.Nx * p.Ny * p.Nz : nullptr;
-    if (!CUBIC && tile_never_clamps<FOCUS>(p, bx, by, bz))
+    if (tile_never_clamps<FOCUS>(p, bx, by, bz))
         hercules_tile<CUBIC, FOCUS, CF, SINAPO, IQK, false>(iq, iq_inv, p, bx, by, bz, gt_lo, gt_hi, vol, pp, win,
                                                             wlo_s, wmode_s, woff_s);
     else
@@ -579,7 +603,7 @@ __global__ __launch_bounds__(256, 4) void hercules_batch_kernel(const BatchFrame
         const int wz = tile / per_z, wxy = tile - wz * per_z;
         const int bx = (wxy % fr.nxt) * HT_X, by = (wxy / fr.nxt) * HT_Y, bz = (fr.nzt - 1 - wz) * HT_Z;
         const int G = p.readi_groups > 1 ? p.readi_groups : 1;
-        if (!CUBIC && tile_never_clamps<FOCUS>(p, bx, by, bz))
+        if (tile_never_clamps<FOCUS>(p, bx, by, bz))
             hercules_tile<CUBIC, FOCUS, CF, SINAPO, IQK, false>(fr.iq, fr.iq_inv, p, bx, by, bz, 0, G * p.T, fr.vol,
                                                                 nullptr, win, wlo_s, wmode_s, woff_s);
         else
