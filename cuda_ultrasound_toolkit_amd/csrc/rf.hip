@@ -134,6 +134,7 @@ constexpr int DEC_NT = 64;      // samples per workgroup
 constexpr int DEC_KC = 128;     // k rows staged per chunk
 constexpr int DEC_ROWB = 144;   // LDS bytes per sample column of a byte tile (16-byte padded)
 constexpr int DEC_ROWF = 65;    // LDS floats per k row of the fp32 tile (bank padded)
+constexpr int DEC_STREAM_WG_PER_CU = 4;  // resident workgroups of the streaming int8 decode
 
 // Load 8 consecutive samples (k row, sample s) as int32 values.
 template <typename T, bool VEC>
@@ -236,6 +237,105 @@ __global__ __launch_bounds__(256) void decode_i8_kernel(const int16_t* __restric
     }
 }
 
+// Streaming form of decode_i8_kernel for T <= 128 (one k chunk) and 16-byte
+// aligned rows: a fixed grid of workgroups walks the (channel, 64-sample)
+// tiles in order (tile = blockIdx.x + i * gridDim.x), keeps H in registers,
+// and issues the next tile's int16 loads before the current tile's MFMA and
+// stores, so HBM reads stay in flight while a workgroup computes and writes.
+// Results are identical to decode_i8_kernel (same products, same integer sums).
+template <int OUT_STRIDE>
+__global__ __launch_bounds__(256) void decode_i8_stream_kernel(const int16_t* __restrict__ raw,
+                                                               const int16_t* __restrict__ map, size_t R0, int T,
+                                                               int C, int S, int Kp, int nsx,
+                                                               const int8_t* __restrict__ h8,
+                                                               const int* __restrict__ rowsum, float scale,
+                                                               float* __restrict__ out)
+{
+    __shared__ __attribute__((aligned(16))) int8_t bhi[DEC_NT * DEC_ROWB];
+    __shared__ __attribute__((aligned(16))) int8_t blo[DEC_NT * DEC_ROWB];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int mbase = wave * 32;
+    const bool mvalid = mbase < Kp;
+    const int ntiles = nsx * C;
+    const int kq = tid >> 3, sg = tid & 7;
+
+    // A operand (H rows mbase.., 32 k per fragment) for the whole K
+    v4i afr[DEC_KC / 32];
+#pragma unroll
+    for (int kk = 0; kk < DEC_KC / 32; ++kk) {
+        afr[kk] = v4i{0, 0, 0, 0};
+        if (mvalid && kk * 32 < Kp)
+            afr[kk] = *(const v4i*)(h8 + (size_t)(mbase + (lane & 31)) * Kp + kk * 32 + (lane >> 5) * 16);
+    }
+
+    // rows kq*4 + j, samples sg*8 .. sg*8+7 of a tile, as packed int16 pairs
+    v4i cur[4];
+    auto load_tile = [&](int tile) {
+        const int c = tile / nsx, sx = tile - c * nsx;
+        const int16_t* src = raw + (size_t)(uint16_t)map[c] * R0;
+        const int s = sx * DEC_NT + sg * 8;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int k = kq * 4 + j;
+            cur[j] = (k < T && s < S) ? *(const v4i*)(src + (size_t)k * S + s) : v4i{0, 0, 0, 0};
+        }
+    };
+    int tile = blockIdx.x;
+    if (tile < ntiles) load_tile(tile);
+    for (; tile < ntiles; tile += gridDim.x) {
+        const int c = tile / nsx, s0 = (tile - c * nsx) * DEC_NT;
+        __syncthreads();  // the previous tile's MFMA reads of LDS are done
+        if (kq * 4 < Kp) {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                unsigned hi = 0, lo = 0;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const unsigned x = ((unsigned)cur[j][e >> 1] >> (16 * (e & 1))) & 0xffffu;
+                    hi |= ((x >> 8) & 0xffu) << (8 * j);
+                    lo |= ((x & 0xffu) ^ 0x80u) << (8 * j);
+                }
+                const int off = (sg * 8 + e) * DEC_ROWB + kq * 4;
+                *(unsigned*)&bhi[off] = hi;
+                *(unsigned*)&blo[off] = lo;
+            }
+        }
+        const int next = tile + gridDim.x;
+        if (next < ntiles) load_tile(next);
+        __syncthreads();
+        if (!mvalid) continue;
+        v16i acc_hi[2], acc_lo[2];
+#pragma unroll
+        for (int n = 0; n < 2; ++n)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) { acc_hi[n][r] = 0; acc_lo[n][r] = 0; }
+#pragma unroll
+        for (int kk = 0; kk < DEC_KC / 32; ++kk) {
+            if (kk * 32 >= Kp) break;
+#pragma unroll
+            for (int nt = 0; nt < 2; ++nt) {
+                const int off = (nt * 32 + (lane & 31)) * DEC_ROWB + kk * 32 + (lane >> 5) * 16;
+                const v4i bh = *(const v4i*)&bhi[off];
+                const v4i bl = *(const v4i*)&blo[off];
+                acc_hi[nt] = __builtin_amdgcn_mfma_i32_32x32x32_i8(afr[kk], bh, acc_hi[nt], 0, 0, 0);
+                acc_lo[nt] = __builtin_amdgcn_mfma_i32_32x32x32_i8(afr[kk], bl, acc_lo[nt], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt) {
+            const int s = s0 + nt * 32 + (lane & 31);
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int m = mbase + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                if (m < T && s < S) {
+                    const int total = 256 * acc_hi[nt][r] + acc_lo[nt][r] + 128 * rowsum[m];
+                    out[(((size_t)m * C + c) * S + s) * OUT_STRIDE] = (float)total * scale;
+                }
+            }
+        }
+    }
+}
+
 template <typename TIN, bool VEC, int OUT_STRIDE>
 __global__ __launch_bounds__(256) void decode_f32_kernel(const TIN* __restrict__ raw,
                                                          const int16_t* __restrict__ map, size_t R0, int T,
@@ -317,7 +417,23 @@ bool launch_decode(const void* raw, int type, const int16_t* d_map, size_t R0, i
     const int Kp = (T + 31) / 32 * 32;
     dim3 grid((S + DEC_NT - 1) / DEC_NT, C, (Kp + 127) / 128);
     const bool vec = ((uintptr_t)raw % 16 == 0) && (R0 % 8 == 0) && (S % 8 == 0);
-    if (type == TYPE_I16 && mode == 1) {
+    if (type == TYPE_I16 && mode == 1 && vec && Kp <= DEC_KC) {
+        // streaming form: a fixed grid of workgroups per CU walks the tiles
+        static int cus = 0;
+        if (cus == 0) {
+            int dev = 0;
+            CUT_HIP_CHECK(hipGetDevice(&dev));
+            CUT_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        }
+        const int nsx = (S + DEC_NT - 1) / DEC_NT;
+        const int ntiles = nsx * C;
+        const int wgs = std::min(ntiles, cus * DEC_STREAM_WG_PER_CU);
+        const int16_t* r = (const int16_t*)raw;
+        if (out_stride == 1)
+            decode_i8_stream_kernel<1><<<wgs, 256, 0, st>>>(r, d_map, R0, T, C, S, Kp, nsx, d_h8, d_rowsum, scale, out);
+        else
+            decode_i8_stream_kernel<2><<<wgs, 256, 0, st>>>(r, d_map, R0, T, C, S, Kp, nsx, d_h8, d_rowsum, scale, out);
+    } else if (type == TYPE_I16 && mode == 1) {
         const int16_t* r = (const int16_t*)raw;
 #define CUT_DEC_I8(V, O) decode_i8_kernel<V, O><<<grid, 256, 0, st>>>(r, d_map, R0, T, C, S, Kp, d_h8, d_rowsum, scale, out)
         if (vec) { if (out_stride == 1) CUT_DEC_I8(true, 1); else CUT_DEC_I8(true, 2); }
