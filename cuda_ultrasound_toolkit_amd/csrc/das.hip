@@ -428,9 +428,22 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
                     if (!chan(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(my_ex), cc)), rho2, apo_c))
                         continue;
                     const char* wbytes = reinterpret_cast<const char*>(win);
+                    // Taps outside the aperture get weight 0, or, with the coherence
+                    // factor, are masked off (and skipped when no lane of the wave is
+                    // inside), which measured faster there (and slower without it).
+                    // chan() already required apo_c > 0.1.
 #pragma unroll
                     for (int j = 0; j < HT_Z; ++j) {
-                        const float w = tap_weight<SINAPO>(rho2, kz2[j], apo_c);
+                        float w;
+                        if constexpr (SINAPO) {
+                            w = apo_c;
+                        } else if constexpr (CF) {
+                            const float s = rho2 * kz2[j];
+                            if (!(s < HANN_S_THR)) continue;
+                            w = hann_poly(s);
+                        } else {
+                            w = tap_weight<false>(rho2, kz2[j], apo_c);
+                        }
                         // the window covers every voxel of the tile with a 2-sample margin,
                         // so the index needs no clamp beyond the reference's [1, S-2], and
                         // none at all in tiles whose every index lies inside it
@@ -450,7 +463,7 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
                         }
                         acc[j].x = fmaf(w, v.x, acc[j].x);
                         acc[j].y = fmaf(w, v.y, acc[j].y);
-                        if constexpr (CF) inc[j] = (w != 0.0f) ? fmaf(v.x, v.x, fmaf(v.y, v.y, inc[j])) : inc[j];
+                        if constexpr (CF) inc[j] = fmaf(v.x, v.x, fmaf(v.y, v.y, inc[j]));  // w != 0 here
                     }
                 }
                 // channels whose window did not fit: direct reads
