@@ -35,8 +35,8 @@ constexpr int HW = 256;                         // longest window staged in LDS 
 // LDS window pool of a workgroup: 32 KiB, shared by the channels of a pass at
 // offsets given by a prefix sum of their window lengths.  Each sample i of a
 // window is stored as the coefficients of the interpolant on [i, i+1]:
-// linear, one float4 (p[i], p[i+1] - p[i]); cubic, two float4 holding the
-// Catmull-Rom polynomial c0 + c1 t + c2 t^2 + c3 t^3 (16 channels per pass for
+// linear, one float4 (p[i], p[i+1] - p[i]); cubic, two float4 (in two planes)
+// holding the Catmull-Rom polynomial c0 + c1 t + c2 t^2 + c3 t^3 (16 channels per pass for
 // linear, 8 for cubic).
 constexpr int HPOOL_BYTES = 32768;
 // apo > 0.1  <=>  fn*rho/(2z) < acos(sqrt(0.1))/pi for the Hann window.
@@ -188,10 +188,14 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
                                               float4* __restrict__ part, float4* win, int* wlo_s, int* wmode_s,
                                               int* woff_s)
 {
-    // window entry size (bytes), pool capacity (entries), channels per pass
-    // and staging threads per channel
-    constexpr int WIN_ESZ = CUBIC ? 2 * (int)sizeof(float4) : (int)sizeof(float4);
-    constexpr int HPOOL = HPOOL_BYTES / WIN_ESZ;
+    // window entry stride (bytes), pool capacity (entries), channels per pass
+    // and staging threads per channel.  Cubic entries are split into two
+    // planes of float4 (c0, c1) and (c2, c3), the second HPOOL entries after
+    // the first, so that both reads of a tap have the 16-byte lane stride of
+    // the linear form (a 32-byte stride measured ~1.4 LDS bank-conflict
+    // cycles per read).
+    constexpr int WIN_ESZ = (int)sizeof(float4);
+    constexpr int HPOOL = HPOOL_BYTES / (CUBIC ? 2 * WIN_ESZ : WIN_ESZ);
     constexpr int NCB = CUBIC ? HCB / 2 : HCB;
     constexpr int TPC = 256 / NCB;
     const int tid = threadIdx.x;
@@ -351,7 +355,7 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
                                 // c3 = (p2 - pm1)/2 + 1.5 (p0 - p1); neighbours outside the row
                                 // are clamped into it: the entries that need them are read
                                 // only at t = 0 (sample S-2 at the clamp) or never
-                                float4* w4 = win + 2 * c_off;
+                                float4* w4 = win + c_off;
                                 for (int q = part; q < c_len; q += TPC) {
                                     const int i = c_wlo + q;
                                     const float2 pm = iq_at<IQK>(iq, row + max(i - 1, 0), iq_inv);
@@ -363,8 +367,8 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
                                     const float c2y = fmaf(-0.5f, p2.y, fmaf(2.0f, p1.y, fmaf(-2.5f, p0.y, pm.y)));
                                     const float c3x = fmaf(1.5f, p0.x - p1.x, 0.5f * (p2.x - pm.x));
                                     const float c3y = fmaf(1.5f, p0.y - p1.y, 0.5f * (p2.y - pm.y));
-                                    w4[2 * q] = make_float4(p0.x, p0.y, c1x, c1y);
-                                    w4[2 * q + 1] = make_float4(c2x, c2y, c3x, c3y);
+                                    w4[q] = make_float4(p0.x, p0.y, c1x, c1y);
+                                    w4[HPOOL + q] = make_float4(c2x, c2y, c3x, c3y);
                                 }
                             } else {
                                 // (p[i], p[i+1] - p[i]); the entry of sample S-1 is never read
@@ -454,7 +458,7 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
                         float2 v;
                         if constexpr (CUBIC) {
                             const float4 e0 = *reinterpret_cast<const float4*>(wbytes + addr);
-                            const float4 e1 = *reinterpret_cast<const float4*>(wbytes + addr + 16);
+                            const float4 e1 = *reinterpret_cast<const float4*>(wbytes + addr + HPOOL * WIN_ESZ);
                             v = make_float2(fmaf(fmaf(fmaf(e1.z, tt, e1.x), tt, e0.z), tt, e0.x),
                                             fmaf(fmaf(fmaf(e1.w, tt, e1.y), tt, e0.w), tt, e0.y));
                         } else {
