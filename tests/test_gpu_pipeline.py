@@ -62,6 +62,27 @@ def test_decode_bit_exact(torch_dev, orc, T, S, encoding):
     assert np.array_equal(c[0::2].reshape(T, Cc, S), outs[2]) and np.all(c[1::2] == 7.0)
 
 
+def test_default_stream_work_is_ordered_before_library_calls(torch_dev, orc):
+    """The library's own stream is ordered with the default stream: an output
+    filled on torch's default stream right before a device-pointer call is
+    never written after the library's kernels (repeated, fresh buffers)."""
+    import torch
+    print("decode after default-stream fill", flush=True)
+    T, S = 128, 512
+    acq = configs.linear_2d(T=T, S=S, seed=T + S)
+    S_, Cc, T_ = _rf_setup(acq)
+    raw = _t(acq.raw, torch_dev)
+    want = orc.decode(orc.convert(acq.raw, L.TYPE_I16, acq.channel_map, T * S, T, Cc, S),
+                      orc.hadamard(T, L.HADAMARD)).astype(np.float32)
+    assert api.set_decode_mode(2)
+    for _ in range(8):
+        out = torch.full((T * Cc * S,), np.nan, dtype=torch.float32, device=torch_dev)
+        assert api.rf_convert_decode(raw, L.TYPE_I16, out, 1, True)
+        torch.cuda.synchronize()
+        assert np.array_equal(out.cpu().numpy().reshape(T, Cc, S), want)
+    assert api.set_decode_mode(0)
+
+
 def test_decode_full_int16_range(torch_dev, orc):
     """Extreme int16 values (the hi/lo split edges) decode exactly."""
     import torch
@@ -174,6 +195,19 @@ def test_das_hercules_single_tile_no_split(torch_dev, orc):
     print("das hercules unsplit", flush=True)
     acq = configs.rca_small(seed=25, T=16, n=8)
     _das_case(acq, orc, torch_dev, True, 1e-4)
+
+
+@pytest.mark.parametrize("cubic", [False, True])
+@pytest.mark.parametrize("nz", [3, 12])
+def test_das_hercules_long_windows(torch_dev, orc, cubic, nz):
+    """Coarse depth grids over 3-14 mm: with 3 planes every window is longer
+    than the LDS limit (all channels read HBM directly); with 12 the windows
+    fit one by one but a pass's windows overflow the LDS pool, so some
+    channels are staged and the rest read directly."""
+    print(f"das hercules long windows nz={nz} cubic={cubic}", flush=True)
+    acq = configs.rca_small(seed=26)
+    acq.bp.output_points[2] = nz
+    _das_case(acq, orc, torch_dev, cubic, 1e-4)
 
 
 def test_das_hercules_sin_apodization(torch_dev, orc):
