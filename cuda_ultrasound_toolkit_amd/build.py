@@ -16,6 +16,10 @@ HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 OBJ = HERE / "_build"
 LIB = HERE / "libcuda_toolkit.so"
+INCLUDE = HERE.parent / "include"
+# out-of-process transfer protocol: server (links the library) and client (plain C)
+TRANSFER_SERVER = HERE / "cut_transfer_server"
+TRANSFER_CLIENT = HERE / "libcut_transfer.so"
 ARCH = os.environ.get("CUT_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
@@ -61,9 +65,31 @@ def build(verbose: bool = True, incremental: bool = False) -> Path:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_transfer()
     if verbose:
-        print(f"built {LIB} from {len(sources)} sources")
+        print(f"built {LIB} from {len(sources)} sources, {TRANSFER_SERVER.name} and {TRANSFER_CLIENT.name}")
     return LIB
+
+
+def _run(cmd, what):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"{what} failed:\n{r.stderr}")
+
+
+def build_transfer() -> None:
+    """The transfer server (C++, linked against the library, rpath $ORIGIN)
+    and the client library (C, gcc, no GPU dependency)."""
+    tdir = CSRC / "transfer"
+    tmp = TRANSFER_SERVER.with_name(TRANSFER_SERVER.name + ".tmp")
+    _run([HIPCC, "-O2", "-std=c++20", "-Wall", f"-I{INCLUDE}", f"-I{tdir}", str(tdir / "transfer_server.cpp"),
+          "-o", str(tmp), f"-L{HERE}", "-lcuda_toolkit", "-Wl,-rpath,$ORIGIN"], "transfer server build")
+    os.replace(tmp, TRANSFER_SERVER)
+    gcc = shutil.which("gcc") or "gcc"
+    tmp = TRANSFER_CLIENT.with_name(TRANSFER_CLIENT.name + ".tmp")
+    _run([gcc, "-O2", "-std=c11", "-Wall", "-fPIC", "-shared", "-fvisibility=hidden", f"-I{INCLUDE}", f"-I{tdir}",
+          str(tdir / "transfer_client.c"), "-o", str(tmp), "-lrt"], "transfer client build")
+    os.replace(tmp, TRANSFER_CLIENT)
 
 
 if __name__ == "__main__":

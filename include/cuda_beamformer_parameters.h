@@ -13,6 +13,8 @@
 
 #ifdef __cplusplus
 extern "C" {
+#else
+#include <stdbool.h> /* the structs hold bool fields; C clients include this header too */
 #endif
 
 #define MAX_ACQ_COUNT 256
