@@ -134,7 +134,7 @@ constexpr int DEC_NT = 64;      // samples per workgroup
 constexpr int DEC_KC = 128;     // k rows staged per chunk
 constexpr int DEC_ROWB = 144;   // LDS bytes per sample column of a byte tile (16-byte padded)
 constexpr int DEC_ROWF = 65;    // LDS floats per k row of the fp32 tile (bank padded)
-constexpr int DEC_STREAM_WG_PER_CU = 4;  // resident workgroups of the streaming int8 decode
+constexpr int DEC_STREAM_WG_PER_CU = 3;  // resident workgroups of the streaming int8 decode (<= 168 VGPRs)
 
 // Load 8 consecutive samples (k row, sample s) as int32 values.
 template <typename T, bool VEC>
@@ -244,7 +244,7 @@ __global__ __launch_bounds__(256) void decode_i8_kernel(const int16_t* __restric
 // stores, so HBM reads stay in flight while a workgroup computes and writes.
 // Results are identical to decode_i8_kernel (same products, same integer sums).
 template <int OUT_STRIDE>
-__global__ __launch_bounds__(256) void decode_i8_stream_kernel(const int16_t* __restrict__ raw,
+__global__ __launch_bounds__(256, 3) void decode_i8_stream_kernel(const int16_t* __restrict__ raw,
                                                                const int16_t* __restrict__ map, size_t R0, int T,
                                                                int C, int S, int Kp, int nsx,
                                                                const int8_t* __restrict__ h8,
@@ -304,31 +304,32 @@ __global__ __launch_bounds__(256) void decode_i8_stream_kernel(const int16_t* __
         if (next < ntiles) load_tile(next);
         __syncthreads();
         if (!mvalid) continue;
-        v16i acc_hi[2], acc_lo[2];
-#pragma unroll
-        for (int n = 0; n < 2; ++n)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) { acc_hi[n][r] = 0; acc_lo[n][r] = 0; }
-#pragma unroll
-        for (int kk = 0; kk < DEC_KC / 32; ++kk) {
-            if (kk * 32 >= Kp) break;
-#pragma unroll
-            for (int nt = 0; nt < 2; ++nt) {
-                const int off = (nt * 32 + (lane & 31)) * DEC_ROWB + kk * 32 + (lane >> 5) * 16;
-                const v4i bh = *(const v4i*)&bhi[off];
-                const v4i bl = *(const v4i*)&blo[off];
-                acc_hi[nt] = __builtin_amdgcn_mfma_i32_32x32x32_i8(afr[kk], bh, acc_hi[nt], 0, 0, 0);
-                acc_lo[nt] = __builtin_amdgcn_mfma_i32_32x32x32_i8(afr[kk], bl, acc_lo[nt], 0, 0, 0);
-            }
-        }
+        // one accumulator per column block: H.hi, times 256, then + H.lo
+        // (|H.hi| <= 128 T, so 256 H.hi + H.lo stays far inside int32)
 #pragma unroll
         for (int nt = 0; nt < 2; ++nt) {
+            v16i acc;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[r] = 0;
+            const int col = (nt * 32 + (lane & 31)) * DEC_ROWB + (lane >> 5) * 16;
+#pragma unroll
+            for (int kk = 0; kk < DEC_KC / 32; ++kk) {
+                if (kk * 32 >= Kp) break;
+                acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(afr[kk], *(const v4i*)&bhi[col + kk * 32], acc, 0, 0, 0);
+            }
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[r] *= 256;
+#pragma unroll
+            for (int kk = 0; kk < DEC_KC / 32; ++kk) {
+                if (kk * 32 >= Kp) break;
+                acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(afr[kk], *(const v4i*)&blo[col + kk * 32], acc, 0, 0, 0);
+            }
             const int s = s0 + nt * 32 + (lane & 31);
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 const int m = mbase + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
                 if (m < T && s < S) {
-                    const int total = 256 * acc_hi[nt][r] + acc_lo[nt][r] + 128 * rowsum[m];
+                    const int total = acc[r] + 128 * rowsum[m];
                     out[(((size_t)m * C + c) * S + s) * OUT_STRIDE] = (float)total * scale;
                 }
             }
