@@ -112,6 +112,44 @@ def test_staged_beamform_rca_small(torch_dev, orc, fmt, cf):
     assert err < (3e-4 if fmt == 1 else 3e-3)
 
 
+@pytest.mark.parametrize("fmt", [1, 2])
+@pytest.mark.parametrize("nz", [3, 12])
+@pytest.mark.parametrize("cubic", [False, True])
+def test_staged_long_windows(torch_dev, orc, fmt, nz, cubic):
+    """Staged IQ where windows are too long for LDS (3 depth planes over
+    3-14 mm: the direct reads convert the staged samples) or overflow the LDS
+    pool (12 planes: some channels staged, the rest read directly).  The
+    oracle beamforms the staged samples themselves (unstaged exactly), so the
+    bar is the FP32 one: any mistake in reading or converting them shows."""
+    import torch
+    print(f"staged long windows fmt={fmt} nz={nz} cubic={cubic}", flush=True)
+    acq = configs.rca_small(seed=27)
+    bp = acq.bp
+    bp.output_points[2] = nz
+    S, Cc, T = bp.dec_data_dim[0], bp.dec_data_dim[1], bp.dec_data_dim[2]
+    st = orc.pipeline(acq, cubic=cubic, return_stages=True)
+    assert api.rf_init((bp.rf_raw_dim[0], bp.rf_raw_dim[1]), (S, Cc, T), bp.decode)
+    assert api.rf_set_match_filter(np.array(bp.rf_filter[: bp.filter_length], np.float32))
+    words = torch.zeros(T * Cc * S, dtype=torch.int32, device=torch_dev)
+    ok, scale = api.rf_hilbert_format(_t(st["dec"].astype(np.float32).ravel(), torch_dev), 1, words, fmt)
+    assert ok
+    n = bp.output_points
+    vol = torch.zeros(n[0] * n[1] * n[2] * 2, dtype=torch.float32, device=torch_dev)
+    assert api.set_das_options(cubic=cubic, iq_format=fmt)
+    try:
+        assert api.das_device_format(words, fmt, scale, vol, bp)
+        torch.cuda.synchronize()
+    finally:
+        assert api.set_das_options(cubic=True, cf_hercules=False, iq_format=0)
+        assert api.rf_set_match_filter(None)
+    iq = unstage(words.cpu().numpy().view(np.uint32), fmt, scale).reshape(T, Cc, S)
+    want, _ = orc.das(iq, bp, cubic=cubic)
+    got = vol.cpu().numpy().view(np.complex64).reshape(want.shape)
+    err = rel_l2(got, want)
+    print(f"  volume rel L2 {err:.3e} (against the oracle on the staged IQ)")
+    assert err < 1e-4
+
+
 def test_staged_format_rejected_for_forces(torch_dev):
     import torch
     acq = configs.linear_2d(seed=1)
