@@ -977,7 +977,7 @@ __global__ __launch_bounds__(1024) void tpw_kernel(const float2* __restrict__ iq
 }
 
 // Launch tuning: the transmit split of the single-frame HERCULES launch (the
-// heuristic das_splits, or the fastest of 1, 2, 4 and 8 splits) and the waves
+// heuristic das_splits, or the fastest of 1, 2, 4, 8 and 16 splits) and the waves
 // per voxel group of the per-voxel kernels (vox_waves, or the fastest of 1 to
 // 16), measured the first time a shape is seen (outside stream capture).
 namespace {
@@ -986,7 +986,7 @@ std::mutex g_tune_mutex;
 std::map<TuneKey, int> g_tuned_splits;  // HERCULES transmit splits
 std::map<TuneKey, int> g_tuned_waves;   // waves per voxel group of the per-voxel kernels
 bool g_das_autotune = true;
-constexpr int TUNE_MAX_SPLITS = 8;
+constexpr int TUNE_MAX_SPLITS = 16;
 
 int max_splits(int gt_total) { return std::max(1, std::min(TUNE_MAX_SPLITS, gt_total / 8)); }
 
