@@ -386,7 +386,7 @@ def main():
         "config": {"workload": "rca_3d: 128 tx x 128 ch x 4096 int16 -> decode -> 64-tap filter + Hilbert "
                                f"-> HERCULES DAS 128^3, {args.interp} interpolation, F# 1.5 Hann",
                    "interp": args.interp, "decode": "int8 MFMA hi/lo (bit-exact)"},
-        "roofline": {"bound": "valu", "kernel": "hercules_kernel (DAS)",
+        "roofline": {"bound": "valu", "kernel": "hercules_kernel + hercules_reduce (DAS launch)",
                      "achieved": round(das_tflops, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(das_tflops / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
                      "work": f"{taps.value} active taps x {FLOP_PER_TAP[args.interp]} FLOP",
