@@ -1,6 +1,9 @@
 """Out-of-process round trips through cut_transfer_server and the C client
-library: results must equal the in-process calls of the same entry points
-bit for bit (same library, same inputs)."""
+library against the in-process calls of the same entry points (same library,
+same inputs).  The DAS launch shape is tuned by timing once per process, so
+the server may sum the beamformer's terms in another order than the test
+process: volumes are compared to 1e-6 relative L2; motion maps and
+correlation maps (no tuned launches) bit for bit."""
 import os
 
 import numpy as np
@@ -30,14 +33,14 @@ def test_transfer_beamform_i16_and_f32(server, lib):
     want = api.beamform_volume(acq.raw, acq.bp)
     got = server.beamform(acq.raw, acq.bp)
     assert got is not None and server.last_ok
-    assert np.array_equal(got, want)
+    assert np.linalg.norm(got - want) <= 1e-6 * np.linalg.norm(want)
     print("transfer beamform linear_2d f32", flush=True)
     acq = configs.linear_2d(seed=32)
     acq.bp.data_type = L.TYPE_F32
     raw = acq.raw.astype(np.float32)
     want = api.beamform_volume(raw, acq.bp)
     got = server.beamform(raw, acq.bp)
-    assert got is not None and np.array_equal(got, want)
+    assert got is not None and np.linalg.norm(got - want) <= 1e-6 * np.linalg.norm(want)
 
 
 def test_transfer_motion_and_corr(server, lib):
