@@ -63,6 +63,28 @@ def test_batch_graph_matches_sequential_and_oracle(torch_dev, orc, frames):
     assert api.set_das_options(cubic=True)
 
 
+@pytest.mark.parametrize("cubic,cf", [(True, False), (False, True)])
+def test_batch_graph_variants_match_sequential(torch_dev, orc, frames, cubic, cf):
+    """Cubic interpolation and the coherence factor (applied where the last
+    transmit split of a tile adds the partial sums) in the batch kernel."""
+    import torch
+    print(f"batch: variants cubic={cubic} cf={cf}", flush=True)
+    assert api.set_das_options(cubic=cubic, cf_hercules=cf)
+    try:
+        raws = [_t(a.raw, torch_dev) for a in frames]
+        vg = volumes(frames, torch_dev)
+        vs = volumes(frames, torch_dev)
+        assert api.beamform_batch([(r, v, a.bp) for r, v, a in zip(raws, vg, frames)], mode=0)[0]
+        assert api.beamform_batch([(r, v, a.bp) for r, v, a in zip(raws, vs, frames)], mode=1)[0]
+        torch.cuda.synchronize()
+    finally:
+        assert api.set_das_options(cubic=True, cf_hercules=False)
+    for a, g, s in zip(frames, vg, vs):
+        assert rel_l2(as_complex(g, a.bp), as_complex(s, a.bp)) < 1e-5
+    want = orc.pipeline(frames[2], cubic=cubic, cf_hercules=cf)
+    assert rel_l2(as_complex(vg[2], frames[2].bp), want) < 1e-4
+
+
 def test_batch_graph_replay_reads_new_data(torch_dev, frames):
     """Replaying the captured graph after the raw buffers are refilled gives
     the volumes of the new data."""
