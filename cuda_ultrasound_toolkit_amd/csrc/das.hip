@@ -6,16 +6,18 @@
 //
 // HERCULES design.  A workgroup owns a 16 x 16 x 8 voxel tile; each thread owns
 // one (x, y) column and its 8 voxels along z, so per (t, c) the lateral
-// distance rho^2 = dx^2 + dy^2 (and, for Hann apodization, rho itself) is
-// computed once and shared by 8 voxels.  For every transmit the workgroup
-// derives, from the tile's bounding box and the F-number aperture, the channel
-// range that can have apo > 0.1 anywhere in the tile; channels outside are
-// skipped without touching memory.  Channels inside are processed 16 at a
-// time: for each, the span of sample indices any voxel of the tile can read is
-// bounded from the box (min/max receive and transmit path), and that window
-// (<= 256 complex samples) is copied into LDS, so the per-tap interpolation
-// reads LDS instead of gathering from L2.  A window that does not fit falls
-// back to direct global reads for that channel.
+// distance rho^2 = dx^2 + dy^2 is computed once and shared by 8 voxels (the
+// Hann weight is a polynomial in rho^2 (fn / 2z)^2).  For every transmit the
+// workgroup derives, from the tile's bounding box and the F-number aperture,
+// the channel range that can have apo > 0.1 anywhere in the tile; channels
+// outside are skipped without touching memory.  Channels inside are processed
+// 16 (linear) or 8 (cubic) at a time: for each, the span of sample indices any
+// voxel of the tile can read is bounded from the box (min/max receive and
+// transmit path), and that window (<= 256 samples) is staged into a shared
+// LDS pool as interpolation coefficients per sample, so a tap is one (two)
+// 16-byte LDS reads and a few FMAs instead of gathers from L2.  A window that
+// is too long or does not fit the pool falls back to direct global reads for
+// that channel.
 #include "common.hpp"
 
 #include <hip/hip_fp16.h>
