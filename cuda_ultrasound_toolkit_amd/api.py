@@ -255,7 +255,8 @@ def motion3d_learned(d_params, d_tpl, d_src, stride, d_disp, d_peak, radius=2, t
 # ---- batches of frames ------------------------------------------------------
 def beamform_batch(frames, mode: int = 0, latencies: bool = False):
     """frames: [(d_raw, d_volume, bp)] device tensors and parameters.  mode 0:
-    persistent DAS launch inside a replayed HIP graph; 1: frame by frame.
+    persistent DAS launch inside a replayed HIP graph; 1: frame by frame;
+    2: as 0 with the work kept in frame order (lower per-frame latency).
     Returns (ok, per-frame completion times in ms from the batch start or None)."""
     n = len(frames)
     arr = (L.CutFrame * n)()

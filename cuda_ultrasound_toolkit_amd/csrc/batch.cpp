@@ -154,7 +154,8 @@ extern "C" bool cut_beamform_batch(const CutFrame* frames, int n, int mode, doub
     try {
         std::lock_guard<std::mutex> lock(cut::api_mutex());
         if (!frames || n <= 0) CUT_FAIL("batch: no frames");
-        if (mode != 0 && mode != 1) CUT_FAIL("batch: mode must be 0 (graph) or 1 (sequential)");
+        if (mode < 0 || mode > 2)
+            CUT_FAIL("batch: mode must be 0 (graph), 1 (sequential) or 2 (graph, frames in order)");
         auto& s = batch_state();
         hipStream_t st = cut::stream();
         if (!s.cus) {
@@ -231,6 +232,9 @@ extern "C" bool cut_beamform_batch(const CutFrame* frames, int n, int mode, doub
             sig.push_back(hash_bytes(frames[f].bp, sizeof(CudaBeamformerParameters)));
         }
         shape_sig = mix(shape_sig, (uint64_t)p0.fmt * 131 + p0.opts.cubic * 7 + p0.opts.cf_hercules);
+        // mode 2 keeps the frames in order (each frame's tiles before the next
+        // frame's): early frames complete early; mode 0 takes the fastest order
+        shape_sig = mix(shape_sig, (uint64_t)mode);
         sig.push_back(shape_sig);
 
         if (!s.frames.ensure(std::max(fd.size() * sizeof(cut::BatchFrameDev), s.frames.bytes)) ||
@@ -240,7 +244,7 @@ extern "C" bool cut_beamform_batch(const CutFrame* frames, int n, int mode, doub
 
         // launch shape: tuned once per batch shape
         auto tuned = s.tuned.find(shape_sig);
-        LaunchShape ls = tuned != s.tuned.end() ? tuned->second : LaunchShape{4, 0};
+        LaunchShape ls = tuned != s.tuned.end() ? tuned->second : LaunchShape{4, mode == 2 ? 1 : 0};
         if (s.autotune && tuned == s.tuned.end()) {
             drop_graph(s);
             CUT_HIP_CHECK(hipMemcpy(s.frames.ptr, fd.data(), fd.size() * sizeof(cut::BatchFrameDev),
@@ -250,6 +254,7 @@ extern "C" bool cut_beamform_batch(const CutFrame* frames, int n, int mode, doub
             CUT_HIP_CHECK(hipEventCreate(&a));
             CUT_HIP_CHECK(hipEventCreate(&b));
             for (const LaunchShape& c : CANDIDATES) {
+                if (mode == 2 && c.order != 1) continue;
                 if (!upload_items(s, make_items(s, n, c.order))) return false;
                 int* cnt = s.counters.as<int>();
                 long long* tm = s.times.as<long long>();

@@ -66,13 +66,15 @@ typedef struct CutFrame {
  * persistent HERCULES DAS launch over the work of the whole batch, captured
  * into a HIP graph that is replayed while pointers, sizes and options stay the
  * same (frames must be HERCULES and share focus type and apodization); mode
- * 1: the single-frame path frame after frame.  Each frame keeps its own RF
+ * 1: the single-frame path frame after frame; mode 2: as mode 0 with the
+ * frames' work kept in frame order (lower per-frame latency, the launch grid
+ * still tuned).  Each frame keeps its own RF
  * state and buffers across calls.  frame_ms (may be NULL): per frame, the time
  * from the start of the batch on the GPU to that frame's completion. */
 CUT_EXT_EXPORT bool cut_beamform_batch(const CutFrame* frames, int n, int mode, double* frame_ms);
 /* Autotuning, on by default: the batch DAS launch shape (grid size, work
- * order) per batch shape, the single-frame HERCULES transmit split (1, 2, 4
- * or 8 workgroups per tile, partial sums added in split order) and, for
+ * order) per batch shape, the single-frame HERCULES transmit split (1, 2, 4,
+ * 8 or 16 workgroups per tile, partial sums added in split order) and, for
  * FORCES, UFORCES and RCA_TPW, the waves per 64-voxel workgroup (1 to 16,
  * each taking a share of the voxel's outer loop) per frame shape; each is
  * timed on first use (that call synchronises) and the fastest kept. */

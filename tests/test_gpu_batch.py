@@ -111,6 +111,26 @@ def test_batch_graph_replay_reads_new_data(torch_dev, frames):
     assert api.set_das_options(cubic=True)
 
 
+def test_batch_graph_in_frame_order(torch_dev, frames):
+    """Mode 2 (graph, work kept in frame order) computes the same volumes as
+    mode 0 and reports a completion time for every frame."""
+    import torch
+    print("batch: graph in frame order", flush=True)
+    assert api.set_das_options(cubic=False)
+    raws = [_t(a.raw, torch_dev) for a in frames]
+    v0 = volumes(frames, torch_dev)
+    v2 = volumes(frames, torch_dev)
+    assert api.beamform_batch([(r, v, a.bp) for r, v, a in zip(raws, v0, frames)], mode=0)[0]
+    ok, ms = api.beamform_batch([(r, v, a.bp) for r, v, a in zip(raws, v2, frames)], mode=2, latencies=True)
+    assert ok and all(m > 0 for m in ms)
+    torch.cuda.synchronize()
+    g, o = api.batch_launch_shape()
+    assert o == 1
+    for a, x, y in zip(frames, v0, v2):
+        assert torch.equal(x, y)  # same tiles, same code: identical bits
+    assert api.set_das_options(cubic=True)
+
+
 def test_batch_rejects_forces_in_graph_mode(torch_dev):
     import torch
     a = configs.linear_2d(seed=1)
