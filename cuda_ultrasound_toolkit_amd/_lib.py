@@ -145,6 +145,15 @@ def load() -> C.CDLL:
     if not LIB_PATH.exists():
         raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() (or python -m "
                            "cuda_ultrasound_toolkit_amd.build) first; there is no fallback path")
+    # torch ships its own libamdhip64.so.7 / libhsa-runtime64.so.1 under the
+    # same sonames as /opt/rocm's, and whichever copy loads first serves the
+    # whole process.  torch does not initialise on a newer runtime loaded
+    # before it (torch.cuda.is_available() reads False), so when torch is
+    # installed it is imported first and the library binds to its runtime.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(str(LIB_PATH))
     B, V, P = C.c_bool, C.c_void_p, C.POINTER
     u = C.c_uint
