@@ -557,9 +557,12 @@ __device__ __forceinline__ bool tile_never_clamps(const DasParams& p, int bx, in
     return ilo >= 2.0f && ihi <= (float)p.S - 3.0f;
 }
 
-// Four workgroups per CU (one wave per SIMD each, four waves per SIMD): the
-// register budget is then 128 VGPRs, which the linear and cubic loops meet
-// without spilling (the coherence-factor variants spill a few bytes).
+// Four workgroups per CU (four waves per SIMD): a budget of 128 VGPRs, which
+// the linear loops meet without spilling.  The cubic loops (two-plane
+// coefficient reads, Horner steps) want ~150 and spill 10-28 registers at
+// 128; with three workgroups per CU they do not spill but ran slower on the
+// 128^3 frame (13.9 against 12.6 ms), so the cubic kernels keep four too.
+
 template <bool CUBIC, bool FOCUS, bool CF, bool SINAPO, int IQK>
 __global__ __launch_bounds__(256, 4) void hercules_kernel(const void* __restrict__ iq, float iq_inv, const DasParams p,
                                                        const HercGrid hg, float2* __restrict__ vol,
