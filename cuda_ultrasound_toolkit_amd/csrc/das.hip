@@ -175,19 +175,21 @@ __device__ __forceinline__ MinMax dist_range(float e, float a, float b)
 // of the (group, transmit) loop, one workgroup each, deepest tiles first (the
 // active-tap count grows with depth, so heavy workgroups start first and the
 // light ones fill the tail).  With splits > 1 every workgroup writes its partial
-// sums (acc.re, acc.im, incoherent) and hercules_reduce adds them in split order.
+// sums and hercules_reduce adds them in split order: (acc.re, acc.im) as a
+// float2 per voxel, or (acc.re, acc.im, incoherent, 0) as a float4 when the
+// coherence factor needs the incoherent sum.
 struct HercGrid {
     int nxt, nyt, nzt, splits, gt_per_split;
 };
 
 // One 16 x 16 x 8 voxel tile at (bx, by, bz) over the (group, transmit)
 // range [gt_lo, gt_hi).  Writes the final voxels (coherence factor applied
-// when CF), or, when part is not null, the partial sums (acc, incoherent) to
-// part[voxel].  All threads of the workgroup must call it (it synchronises).
+// when CF), or, when part is not null, the partial sums to part[voxel]
+// (float2 acc, or float4 (acc, incoherent, 0) when CF).  All threads of the workgroup must call it (it synchronises).
 template <bool CUBIC, bool FOCUS, bool CF, bool SINAPO, int IQK, bool CLAMP>
 __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float iq_inv, const DasParams& p, int bx,
                                               int by, int bz, int gt_lo, int gt_hi, float2* __restrict__ vol,
-                                              float4* __restrict__ part, float4* win, int* wlo_s, int* wmode_s,
+                                              void* __restrict__ part, float4* win, int* wlo_s, int* wmode_s,
                                               int* woff_s)
 {
     // window entry stride (bytes), pool capacity (entries), channels per pass
@@ -502,7 +504,8 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
         const size_t vi = ((size_t)iz * p.Ny + iy) * p.Nx + ix;
         float2 a = acc[j];
         if (part) {
-            part[vi] = make_float4(a.x, a.y, inc[j], 0.0f);
+            if constexpr (CF) static_cast<float4*>(part)[vi] = make_float4(a.x, a.y, inc[j], 0.0f);
+            else static_cast<float2*>(part)[vi] = a;
             continue;
         }
         if constexpr (CF) {
@@ -566,7 +569,7 @@ __device__ __forceinline__ bool tile_never_clamps(const DasParams& p, int bx, in
 template <bool CUBIC, bool FOCUS, bool CF, bool SINAPO, int IQK>
 __global__ __launch_bounds__(256, 4) void hercules_kernel(const void* __restrict__ iq, float iq_inv, const DasParams p,
                                                        const HercGrid hg, float2* __restrict__ vol,
-                                                       float4* __restrict__ part)
+                                                       void* __restrict__ part)
 {
     __shared__ float4 win[HPOOL_BYTES / sizeof(float4)];
     __shared__ int wlo_s[HCB];
@@ -581,7 +584,8 @@ __global__ __launch_bounds__(256, 4) void hercules_kernel(const void* __restrict
     const int G = p.readi_groups > 1 ? p.readi_groups : 1;
     const int gt_lo = split * hg.gt_per_split;
     const int gt_hi = min(G * p.T, gt_lo + hg.gt_per_split);
-    float4* pp = hg.splits > 1 ? part + (size_t)split * p.Nx * p.Ny * p.Nz : nullptr;
+    constexpr size_t PART_ESZ = CF ? sizeof(float4) : sizeof(float2);
+    void* pp = hg.splits > 1 ? static_cast<char*>(part) + (size_t)split * p.Nx * p.Ny * p.Nz * PART_ESZ : nullptr;
     if (tile_never_clamps<FOCUS>(p, bx, by, bz))
         hercules_tile<CUBIC, FOCUS, CF, SINAPO, IQK, false>(iq, iq_inv, p, bx, by, bz, gt_lo, gt_hi, vol, pp, win,
                                                             wlo_s, wmode_s, woff_s);
@@ -648,20 +652,28 @@ __global__ void batch_start_kernel(int* counters, int n, long long* start)
 }
 
 // Sum of the per-split partials in split order (deterministic), then the
-// coherence factor when it applies.
-__global__ __launch_bounds__(256) void hercules_reduce(const float4* __restrict__ part, int splits, size_t nvox,
-                                                       int apply_cf, float cw, float2* __restrict__ vol)
+// coherence factor when it applies (CF: float4 partials with the incoherent
+// sum; otherwise float2).
+template <bool CF>
+__global__ __launch_bounds__(256) void hercules_reduce(const void* __restrict__ part, int splits, size_t nvox,
+                                                       float cw, float2* __restrict__ vol)
 {
     const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= nvox) return;
     float ax = 0.0f, ay = 0.0f, inc = 0.0f;
     for (int s = 0; s < splits; ++s) {
-        const float4 q = part[(size_t)s * nvox + v];
-        ax += q.x;
-        ay += q.y;
-        inc += q.z;
+        if constexpr (CF) {
+            const float4 q = static_cast<const float4*>(part)[(size_t)s * nvox + v];
+            ax += q.x;
+            ay += q.y;
+            inc += q.z;
+        } else {
+            const float2 q = static_cast<const float2*>(part)[(size_t)s * nvox + v];
+            ax += q.x;
+            ay += q.y;
+        }
     }
-    if (apply_cf) {
+    if constexpr (CF) {
         float cf = powf((ax * ax + ay * ay) / inc, cw);
         if (isnan(cf)) cf = 0.0f;
         ax *= cf;
@@ -1085,7 +1097,7 @@ int das_splits(long long tiles, int gt_total)
 namespace {
 template <bool CU, bool FO, bool CFV, bool SA>
 bool launch_herc_fmt(int fmt, dim3 grid, hipStream_t st, const void* iq, float inv, const DasParams& p,
-                     const HercGrid& hg, float2* vol, float4* part)
+                     const HercGrid& hg, float2* vol, void* part)
 {
     if (fmt == IQ_F32) {
         hercules_kernel<CU, FO, CFV, SA, IQ_F32><<<grid, 256, 0, st>>>(iq, inv, p, hg, vol, part);
@@ -1102,7 +1114,7 @@ bool launch_herc_fmt(int fmt, dim3 grid, hipStream_t st, const void* iq, float i
 
 template <bool CU, bool FO, bool CFV>
 bool launch_herc_sa(bool sa, int fmt, dim3 grid, hipStream_t st, const void* iq, float inv, const DasParams& p,
-                    const HercGrid& hg, float2* vol, float4* part)
+                    const HercGrid& hg, float2* vol, void* part)
 {
     return sa ? launch_herc_fmt<CU, FO, CFV, true>(fmt, grid, st, iq, inv, p, hg, vol, part)
               : launch_herc_fmt<CU, FO, CFV, false>(fmt, grid, st, iq, inv, p, hg, vol, part);
@@ -1110,7 +1122,7 @@ bool launch_herc_sa(bool sa, int fmt, dim3 grid, hipStream_t st, const void* iq,
 
 template <bool CU, bool FO>
 bool launch_herc_cf(bool cf, bool sa, int fmt, dim3 grid, hipStream_t st, const void* iq, float inv,
-                    const DasParams& p, const HercGrid& hg, float2* vol, float4* part)
+                    const DasParams& p, const HercGrid& hg, float2* vol, void* part)
 {
     return cf ? launch_herc_sa<CU, FO, true>(sa, fmt, grid, st, iq, inv, p, hg, vol, part)
               : launch_herc_sa<CU, FO, false>(sa, fmt, grid, st, iq, inv, p, hg, vol, part);
@@ -1118,7 +1130,7 @@ bool launch_herc_cf(bool cf, bool sa, int fmt, dim3 grid, hipStream_t st, const 
 
 template <bool CU>
 bool launch_herc_fo(bool fo, bool cf, bool sa, int fmt, dim3 grid, hipStream_t st, const void* iq, float inv,
-                    const DasParams& p, const HercGrid& hg, float2* vol, float4* part)
+                    const DasParams& p, const HercGrid& hg, float2* vol, void* part)
 {
     return fo ? launch_herc_cf<CU, true>(cf, sa, fmt, grid, st, iq, inv, p, hg, vol, part)
               : launch_herc_cf<CU, false>(cf, sa, fmt, grid, st, iq, inv, p, hg, vol, part);
@@ -1219,14 +1231,14 @@ bool launch_das(const void* iq, int iq_fmt, float iq_scale, const DasParams& p, 
         const bool cf = o.cf_hercules != 0;
         const int dflt = das_splits(tiles, G * p.T);
         auto fits = [&](int sp) {
-            return sp <= 1 || (workspace && workspace_bytes >= (size_t)sp * nvox * sizeof(float4));
+            return sp <= 1 || (workspace && workspace_bytes >= (size_t)sp * nvox * (cf ? sizeof(float4) : sizeof(float2)));
         };
         auto run = [&](int splits) -> bool {
             // a choice tuned with a larger workspace falls back to one that fits
             if (!fits(splits)) splits = fits(dflt) ? dflt : 1;
             hg.splits = splits;
             hg.gt_per_split = (G * p.T + splits - 1) / splits;
-            float4* part = splits > 1 ? (float4*)workspace : nullptr;
+            void* part = splits > 1 ? workspace : nullptr;
             if (tiles * splits > 0x7fffffffLL) CUT_FAIL("DAS: grid too large");
             dim3 grid((unsigned)(tiles * splits));
             const bool ok = o.cubic ? launch_herc_fo<true>(focus, cf, sinapo, iq_fmt, grid, st, iq, inv, p, hg, vol, part)
@@ -1234,7 +1246,11 @@ bool launch_das(const void* iq, int iq_fmt, float iq_scale, const DasParams& p, 
             if (!ok) return false;
             CUT_HIP_CHECK(hipGetLastError());
             if (splits > 1)
-                hercules_reduce<<<(unsigned)((nvox + 255) / 256), 256, 0, st>>>(part, splits, nvox, cf ? 1 : 0, p.cw, vol);
+            {
+                const unsigned rg = (unsigned)((nvox + 255) / 256);
+                if (cf) hercules_reduce<true><<<rg, 256, 0, st>>>(part, splits, nvox, p.cw, vol);
+                else hercules_reduce<false><<<rg, 256, 0, st>>>(part, splits, nvox, p.cw, vol);
+            }
             return true;
         };
         const TuneKey key{p.Nx, p.Ny, p.Nz, p.T, p.C, p.S, G, iq_fmt, o.cubic, cf, focus, sinapo};
