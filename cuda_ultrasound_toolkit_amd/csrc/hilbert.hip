@@ -1,4 +1,4 @@
-// Hilbert transform with optional match filter, one row per workgroup slice.
+// Hilbert transform with optional match filter.
 //
 // Semantics: cuda_toolkit/src/rf_processing/rf_ffts/hilbert_handler.cu:132-155
 // and hilbert_kernels.cuh:17-40.  Per row x of length S:
@@ -6,16 +6,19 @@
 //   w_0 = 0.75/S, w_k = 1.5/S (k >= 1, the Nyquist bin included);
 //   y = unnormalised inverse DFT_S(Y).
 //
-// Implementation: a Stockham auto-sort FFT held in LDS.  Each thread owns 16
+// Implementation: Stockham auto-sort FFTs held in LDS.  Each thread owns 16
 // complex values per pass (radix-16 passes, one radix-2/4/8 pass when log2 S
-// is not a multiple of 4), so S = 4096 takes three passes forward and three
-// inverse.  The first forward pass reads the input row straight from HBM
-// (coalesced; optionally converting raw int16/f32 through the channel map, so
-// convert+Hilbert is one kernel when no decode is needed), the spectral weight
-// and filter are applied as the last forward pass stores, and the last inverse
-// pass writes the complex row straight to HBM.  LDS rows are padded by one
-// complex per 16 so the radix-16 transposes are free of bank conflicts.
-// Twiddles come from a table generated in FP64.
+// is not a multiple of 4), so S = 4096 takes three passes per transform.
+// Power-of-two rows are taken in pairs: one forward transform of
+// x_a + i x_b yields both spectra, then one inverse transform per row
+// (three transforms per pair instead of four).  The first forward pass reads
+// the input rows straight from HBM (coalesced; optionally converting raw
+// int16/f32 through the channel map, so convert+Hilbert is one kernel when no
+// decode is needed), the spectral weight and filter are applied while the
+// two spectra are separated, and the last inverse pass writes each complex
+// row straight to HBM.  Other lengths use Bluestein's method on a power-of-two
+// FFT.  LDS rows are padded by one complex per 16 so the radix-16 transposes
+// are free of bank conflicts.  Twiddles come from a table generated in FP64.
 #include "common.hpp"
 
 #include <hip/hip_fp16.h>
@@ -143,6 +146,30 @@ __device__ __forceinline__ float load_in(const HilbertDev& in, int row, int S, i
     }
 }
 
+// Start of a row's input (uniform when a workgroup holds one row, so that
+// the per-element loads take a scalar base and a 32-bit offset) and the
+// element at idx from it.
+template <int KIND>
+__device__ __forceinline__ const void* row_base(const HilbertDev& in, int row, int S)
+{
+    if constexpr (KIND == 0) {
+        return (const float*)in.ptr + (size_t)row * S * in.stride;
+    } else {
+        const int t = row / in.C, c = row - t * in.C;
+        const size_t off = (size_t)(uint16_t)in.map[c] * in.R0 + (size_t)t * S;
+        if constexpr (KIND == 1) return (const int16_t*)in.ptr + off;
+        else return (const float*)in.ptr + off;
+    }
+}
+
+template <int KIND>
+__device__ __forceinline__ float load_at(const void* base, unsigned stride, unsigned idx)
+{
+    if constexpr (KIND == 0) return ((const float*)base)[idx * stride];
+    else if constexpr (KIND == 1) return (float)((const int16_t*)base)[idx];
+    else return ((const float*)base)[idx];
+}
+
 // One Stockham pass over an S-point row held by P = S/16 threads.  The thread
 // holds 16 values: 16/R butterflies of radix R.  Load phase -> (barrier) ->
 // twiddle + DFT + store phase.
@@ -184,87 +211,6 @@ __device__ __forceinline__ uint32_t bf16_bits(float f)
     return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
 }
 
-template <int LOGS, int KIND, bool FILTER>
-__global__ __launch_bounds__(256) void hilbert_kernel(HilbertDev in, int rows, const float2* __restrict__ tw,
-                                                      const float2* __restrict__ F, void* __restrict__ out,
-                                                      int ofmt, float oscale)
-{
-    constexpr int S = 1 << LOGS;
-    constexpr int P = S / 16;
-    constexpr int RPW = P >= 256 ? 1 : 256 / P;     // rows per workgroup
-    constexpr int NP16 = LOGS / 4;                   // radix-16 passes
-    constexpr int RT = 1 << (LOGS % 4);             // trailing radix (1 = none)
-    constexpr int PADDED = S + S / 16;
-    __shared__ float2 buf[RPW * PADDED];
-
-    const int tid = threadIdx.x % P;
-    const int rsub = threadIdx.x / P;
-    const int row = blockIdx.x * RPW + rsub;
-    const bool active = row < rows;
-    float2* lds = buf + rsub * PADDED;
-    const int rrow = active ? row : 0;
-
-    const float scale = 1.5f / (float)S;
-    auto load_global = [&](int idx) { return make_float2(active ? load_in<KIND>(in, rrow, S, idx) : 0.0f, 0.0f); };
-    auto load_lds = [&](int idx) { return lds[lds_idx(idx)]; };
-    auto store_lds = [&](int idx, float2 v) { lds[lds_idx(idx)] = v; };
-    auto store_spec = [&](int k, float2 v) {
-        if (k > S / 2) {
-            v = make_float2(0.0f, 0.0f);
-        } else {
-            const float w = k == 0 ? scale * 0.5f : scale;
-            v = make_float2(v.x * w, v.y * w);
-            if constexpr (FILTER) v = cmul(v, F[k]);
-        }
-        lds[lds_idx(k)] = v;
-    };
-    const size_t obase = (size_t)rrow * S;
-    auto store_out = [&](int idx, float2 v) {
-        if (!active) return;
-        if (ofmt == IQ_F32) {
-            static_cast<float2*>(out)[obase + idx] = v;
-        } else if (ofmt == IQ_F16) {
-            static_cast<__half2*>(out)[obase + idx] = __floats2half2_rn(v.x * oscale, v.y * oscale);
-        } else {
-            static_cast<uint32_t*>(out)[obase + idx] = bf16_bits(v.x) | (bf16_bits(v.y) << 16);
-        }
-    };
-
-    // forward
-    int p = 1;
-#pragma unroll
-    for (int q = 0; q < NP16; ++q) {
-        const bool last = (q == NP16 - 1) && RT == 1;
-        if (q == 0) {
-            if (last) stockham_pass<S, 16, false>(tid, p, tw, load_global, store_spec, false);
-            else stockham_pass<S, 16, false>(tid, p, tw, load_global, store_lds, false);
-        } else {
-            if (last) stockham_pass<S, 16, false>(tid, p, tw, load_lds, store_spec, true);
-            else stockham_pass<S, 16, false>(tid, p, tw, load_lds, store_lds, true);
-        }
-        __syncthreads();
-        p *= 16;
-    }
-    if constexpr (RT > 1) {
-        if (NP16 == 0) stockham_pass<S, RT, false>(tid, p, tw, load_global, store_spec, false);
-        else stockham_pass<S, RT, false>(tid, p, tw, load_lds, store_spec, true);
-        __syncthreads();
-    }
-    // inverse
-    p = 1;
-#pragma unroll
-    for (int q = 0; q < NP16; ++q) {
-        const bool last = (q == NP16 - 1) && RT == 1;
-        if (last) stockham_pass<S, 16, true>(tid, p, tw, load_lds, store_out, true);
-        else {
-            stockham_pass<S, 16, true>(tid, p, tw, load_lds, store_lds, true);
-            __syncthreads();
-        }
-        p *= 16;
-    }
-    if constexpr (RT > 1) stockham_pass<S, RT, true>(tid, p, tw, load_lds, store_out, true);
-}
-
 // All passes of one N-point transform held in LDS.  The first pass reads
 // through first_load (global memory, or LDS when first_lds), the last writes
 // through last_store; a barrier follows every pass whose output later passes
@@ -297,6 +243,99 @@ __device__ __forceinline__ void fft_passes(int tid, const float2* __restrict__ t
         else stockham_pass<N, RT, INV>(tid, p, tw, load_lds, last_store, true);
         if (!last_global) __syncthreads();
     }
+}
+
+// Two rows a, b per transform (power-of-two S): one forward FFT of the
+// complex row z = x_a + i x_b gives both real rows' spectra,
+//   X_a[k] = (Z[k] + conj Z[S-k]) / 2,   X_b[k] = -i (Z[k] - conj Z[S-k]) / 2,
+// so a pair costs one forward and two inverse transforms instead of two and
+// two.  Only k = 0..S/2 are formed (the weight is 0 above); the inverse's
+// first pass reads zeros for k > S/2.  Y_a goes back into the row buffer,
+// Y_b into a second half-length buffer: each index k <= S/2 is read (as Z[k]
+// or Z[S-k]) only by the thread that overwrites it, so the split needs no
+// barrier of its own.  A lone last row (odd row count) pairs with zeros.
+// Rounding of the shared transform scales with the larger row of the pair
+// (FP32, ~1e-7 of its magnitude).
+template <int LOGS, int KIND, bool FILTER>
+__global__ __launch_bounds__(256) void hilbert_pair_kernel(HilbertDev in, int rows, const float2* __restrict__ tw,
+                                                           const float2* __restrict__ F, void* __restrict__ out,
+                                                           int ofmt, float oscale)
+{
+    constexpr int S = 1 << LOGS;
+    constexpr int H = S / 2;
+    constexpr int P = S / 16;
+    constexpr int RPW = P >= 256 ? 1 : 256 / P;     // row pairs per workgroup
+    constexpr int PADDED = S + S / 16;
+    constexpr int PADDED2 = H + H / 16 + 1;          // lds_idx(H) + 1
+    __shared__ float2 buf[RPW * PADDED];
+    __shared__ float2 buf2[RPW * PADDED2];
+
+    const int tid = threadIdx.x % P;
+    const int rsub = threadIdx.x / P;
+    const int pair = blockIdx.x * RPW + rsub;
+    const int ra = 2 * pair, rb = 2 * pair + 1;
+    const bool act_a = ra < rows, act_b = rb < rows;
+    const int rra = act_a ? ra : 0, rrb = act_b ? rb : 0;
+    float2* lds = buf + rsub * PADDED;
+    float2* lds2 = buf2 + rsub * PADDED2;
+
+    const void* base_a = row_base<KIND>(in, rra, S);
+    const void* base_b = row_base<KIND>(in, rrb, S);
+    const unsigned istride = (unsigned)in.stride;
+    auto load_pair = [&](int idx) {
+        return make_float2(act_a ? load_at<KIND>(base_a, istride, (unsigned)idx) : 0.0f,
+                           act_b ? load_at<KIND>(base_b, istride, (unsigned)idx) : 0.0f);
+    };
+    auto store_lds = [&](int idx, float2 v) { lds[lds_idx(idx)] = v; };
+    fft_passes<LOGS, false>(tid, tw, lds, load_pair, false, store_lds, false);
+
+    // split into the two one-sided spectra, weight (w_k / 2) and filter
+    const float half_w = 0.75f / (float)S;
+    for (int k = tid; k <= H; k += P) {
+        const float2 zk = lds[lds_idx(k)];
+        const float2 zn = lds[lds_idx((S - k) & (S - 1))];
+        const float w = k == 0 ? 0.5f * half_w : half_w;
+        float2 ya = make_float2((zk.x + zn.x) * w, (zk.y - zn.y) * w);
+        float2 yb = make_float2((zk.y + zn.y) * w, (zn.x - zk.x) * w);
+        if constexpr (FILTER) {
+            const float2 f = F[k];
+            ya = cmul(ya, f);
+            yb = cmul(yb, f);
+        }
+        lds[lds_idx(k)] = ya;
+        lds2[lds_idx(k)] = yb;
+    }
+    __syncthreads();
+
+    // output rows addressed from (uniform) row starts with 32-bit offsets
+    const size_t esz = ofmt == IQ_F32 ? sizeof(float2) : sizeof(uint32_t);
+    char* const out_a = static_cast<char*>(out) + (size_t)rra * S * esz;
+    char* const out_b = static_cast<char*>(out) + (size_t)rrb * S * esz;
+    auto store_row = [&](bool act, char* orow, int idx, float2 v) {
+        if (!act) return;
+        if (ofmt == IQ_F32) {
+            reinterpret_cast<float2*>(orow)[(unsigned)idx] = v;
+        } else if (ofmt == IQ_F16) {
+            reinterpret_cast<__half2*>(orow)[(unsigned)idx] = __floats2half2_rn(v.x * oscale, v.y * oscale);
+        } else {
+            reinterpret_cast<uint32_t*>(orow)[(unsigned)idx] = bf16_bits(v.x) | (bf16_bits(v.y) << 16);
+        }
+    };
+    auto load_ya = [&](int idx) { return idx > H ? make_float2(0.0f, 0.0f) : lds[lds_idx(idx)]; };
+    auto load_yb = [&](int idx) { return idx > H ? make_float2(0.0f, 0.0f) : lds2[lds_idx(idx)]; };
+    auto store_a = [&](int idx, float2 v) { store_row(act_a, out_a, idx, v); };
+    auto store_b = [&](int idx, float2 v) { store_row(act_b, out_b, idx, v); };
+    // Each transform reads the twiddles through its own opaque copy of the
+    // table pointer, so the compiler reloads them instead of keeping one
+    // transform's twiddle registers live through the next (196 VGPRs, two
+    // workgroups per CU, without; 162 and three with).
+    const float2* tw_a = tw;
+    __asm__ volatile("" : "+s"(tw_a));
+    fft_passes<LOGS, true>(tid, tw_a, lds, load_ya, true, store_a, true);
+    __syncthreads();  // the last pass of a read the row buffer that b's first pass writes
+    const float2* tw_b = tw;
+    __asm__ volatile("" : "+s"(tw_b));
+    fft_passes<LOGS, true>(tid, tw_b, lds, load_yb, false, store_b, true);
 }
 
 // Hilbert transform of rows of any length S (not a power of two) by
@@ -367,10 +406,19 @@ __global__ __launch_bounds__(512) void hilbert_bluestein_kernel(HilbertDev in, i
             static_cast<uint32_t*>(out)[obase + n] = bf16_bits(y.x) | (bf16_bits(y.y) << 16);
         }
     };
-    fft_passes<LOGM, false>(tid, tw, lds, load_a, false, store_V, false);
-    fft_passes<LOGM, true>(tid, tw, lds, load_lds, true, store_mid, false);
-    fft_passes<LOGM, false>(tid, tw, lds, load_lds, true, store_W, false);
-    fft_passes<LOGM, true>(tid, tw, lds, load_lds, true, store_out, true);
+    // each transform through its own opaque copy of the twiddle pointer (see
+    // hilbert_pair_kernel): no twiddle registers live from one to the next
+    const float2* tw1 = tw;
+    fft_passes<LOGM, false>(tid, tw1, lds, load_a, false, store_V, false);
+    const float2* tw2 = tw;
+    __asm__ volatile("" : "+s"(tw2));
+    fft_passes<LOGM, true>(tid, tw2, lds, load_lds, true, store_mid, false);
+    const float2* tw3 = tw;
+    __asm__ volatile("" : "+s"(tw3));
+    fft_passes<LOGM, false>(tid, tw3, lds, load_lds, true, store_W, false);
+    const float2* tw4 = tw;
+    __asm__ volatile("" : "+s"(tw4));
+    fft_passes<LOGM, true>(tid, tw4, lds, load_lds, true, store_out, true);
 }
 
 // Bluestein tables in FP64: chirp[n] = exp(-i pi (n^2 mod 2S) / S), n < S.
@@ -515,11 +563,12 @@ static bool launch_hilbert_s(const HilbertInput& in, int rows, const float2* tw,
 {
     constexpr int S = 1 << LOGS;
     constexpr int P = S / 16;
-    constexpr int RPW = P >= 256 ? 1 : 256 / P;
+    constexpr int RPW = P >= 256 ? 1 : 256 / P;     // row pairs per workgroup
     HilbertDev d{in.ptr, in.stride, in.map, in.R0, in.C};
-    dim3 grid((rows + RPW - 1) / RPW);
+    const int pairs = (rows + 1) / 2;
+    dim3 grid((pairs + RPW - 1) / RPW);
     dim3 block(P * RPW);
-#define CUT_HIL(K, FL) hilbert_kernel<LOGS, K, FL><<<grid, block, 0, st>>>(d, rows, tw, F, out, ofmt, oscale)
+#define CUT_HIL(K, FL) hilbert_pair_kernel<LOGS, K, FL><<<grid, block, 0, st>>>(d, rows, tw, F, out, ofmt, oscale)
     if (F) {
         if (in.kind == 0) CUT_HIL(0, true); else if (in.kind == 1) CUT_HIL(1, true); else CUT_HIL(2, true);
     } else {
