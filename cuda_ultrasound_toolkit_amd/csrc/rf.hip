@@ -59,10 +59,17 @@ bool launch_convert(const void* raw, int type, const int16_t* d_map, size_t R0, 
 // ---------------------------------------------------------------------------
 // A workgroup owns BP_SEG outputs of one (t, c) row.  It demodulates the
 // input samples those outputs need (the segment plus the filter's reach on
-// both sides, clipped to the row) into LDS, then each output is the FIR over
-// the valid taps, divided by their count.  The phase s fd/fs is reduced to
-// revolutions in FP64 and evaluated with v_sin/v_cos.
+// both sides) into LDS, then each output is the FIR over the taps that fall
+// inside the row, divided by their count (taps are finite).  The phase s fd/fs is reduced to
+// revolutions in FP64 and evaluated with v_sin/v_cos.  LDS holds the samples
+// from org = s0 - L/2 on, zeros where the window leaves the row, so every
+// output sums all L taps (the zeros add nothing) in tap order, as the valid
+// taps alone would be summed.  The samples sit in four phase planes, sample
+// org + j at plane j % 4, entry j / 4.  Each thread computes four consecutive
+// outputs, sliding a four-sample register window along the row: four taps
+// per step, one read per plane, consecutive entries across the lanes.
 constexpr int BP_SEG = 1024;
+constexpr int BP_VTS = (BP_SEG + MAX_FILTER_LENGTH) / 4 + 1;  // entries per phase plane
 
 template <typename T>
 __global__ __launch_bounds__(256) void convert_demod_kernel(const T* __restrict__ raw, const int16_t* __restrict__ map,
@@ -70,44 +77,94 @@ __global__ __launch_bounds__(256) void convert_demod_kernel(const T* __restrict_
                                                             const float* __restrict__ h, int L,
                                                             float2* __restrict__ out)
 {
-    __shared__ float2 v[BP_SEG + MAX_FILTER_LENGTH];
-    __shared__ float hr[MAX_FILTER_LENGTH];  // taps in application order: hr[i] = h[L-1-i]
+    static_assert(BP_SEG == 4 * 256, "four outputs per thread");
+    __shared__ float2 vt[4 * BP_VTS];
+    __shared__ __align__(16) float hr[MAX_FILTER_LENGTH];  // taps in application order: hr[i] = h[L-1-i]
     const int row = blockIdx.x;              // t * C + c
     const int t = row / C, c = row - t * C;
     const int s0 = blockIdx.y * BP_SEG;
     const int half = L > 0 ? L / 2 : 0;
-    const int lo = max(0, s0 - half);
-    const int hi = min(S - 1, s0 + BP_SEG - 1 + (L > 0 ? L - 1 - half : 0));
+    const int org = s0 - half;
+    const int nwin = BP_SEG + (L > 0 ? L - 1 : 0);  // window samples org .. org + nwin - 1
+    const int tid = threadIdx.x;
+    // sample org + j of the row (j >= 0)
+    auto v = [&](int j) -> float2& { return vt[(j & 3) * BP_VTS + (j >> 2)]; };
     const T* r = raw + (size_t)(uint16_t)map[c] * R0 + 2 * (size_t)t * S;
-    for (int j = lo + (int)threadIdx.x; j <= hi; j += blockDim.x) {
-        const float I = static_cast<float>(r[2 * (size_t)j]), Q = static_cast<float>(r[2 * (size_t)j + 1]);
-        const double ph = (double)j * fd_fs;
-        const float f = (float)(ph - floor(ph));
-        const float cs = __builtin_amdgcn_cosf(f), sn = __builtin_amdgcn_sinf(f);
-        v[j - lo] = make_float2(I * cs - Q * sn, -I * sn - Q * cs);  // (I - iQ) exp(-2 pi i f)
-    }
-    for (int i = threadIdx.x; i < L; i += blockDim.x) hr[i] = h[L - 1 - i];
-    __syncthreads();
-    for (int k = threadIdx.x; k < BP_SEG; k += blockDim.x) {
-        const int s = s0 + k;
-        if (s >= S) break;
-        float2 y;
-        if (L <= 0) {
-            y = v[s - lo];
-        } else {
-            // taps i with 0 <= s + i - half < S
-            const int ia = max(0, half - s), ib = min(L - 1, S - 1 - s + half);
-            const float2* vs = v + (s - half - lo);
-            float ax = 0.0f, ay = 0.0f;
-            for (int i = ia; i <= ib; ++i) {
-                const float2 q = vs[i];
-                ax = fmaf(q.x, hr[i], ax);
-                ay = fmaf(q.y, hr[i], ay);
-            }
-            const float inv = 1.0f / (float)(ib - ia + 1);
-            y = make_float2(ax * inv, ay * inv);
+    // every thread issues all its loads first
+    constexpr int FILL = (BP_SEG + MAX_FILTER_LENGTH) / 256;
+    float Iv[FILL], Qv[FILL];
+#pragma unroll
+    for (int u = 0; u < FILL; ++u) {
+        const int js = org + tid + 256 * u;  // sample index in the row
+        Iv[u] = Qv[u] = 0.0f;
+        if (tid + 256 * u < nwin && js >= 0 && js < S) {
+            Iv[u] = static_cast<float>(r[2 * (size_t)js]);
+            Qv[u] = static_cast<float>(r[2 * (size_t)js + 1]);
         }
-        out[(size_t)row * S + s] = y;
+    }
+#pragma unroll
+    for (int u = 0; u < FILL; ++u) {
+        const int j = tid + 256 * u, js = org + j;
+        if (j < nwin) {
+            float2 y = make_float2(0.0f, 0.0f);
+            if (js >= 0 && js < S) {
+                const double ph = (double)js * fd_fs;
+                const float f = (float)(ph - floor(ph));
+                const float cs = __builtin_amdgcn_cosf(f), sn = __builtin_amdgcn_sinf(f);
+                const float I = Iv[u], Q = Qv[u];
+                y = make_float2(I * cs - Q * sn, -I * sn - Q * cs);  // (I - iQ) exp(-2 pi i f)
+            }
+            v(j) = y;
+        }
+    }
+    for (int i = tid; i < L; i += blockDim.x) hr[i] = h[L - 1 - i];
+    __syncthreads();
+    const int sb = s0 + 4 * tid;
+    if (sb >= S) return;
+    float2* const orow = out + (size_t)row * S;
+    const int nout = min(4, S - sb);
+    if (L <= 0) {
+        for (int q = 0; q < nout; ++q) orow[sb + q] = v(4 * tid + q);
+        return;
+    }
+    // output sb + q takes sample org + 4 tid + q + i at tap i
+    const float2* p0 = vt + 0 * BP_VTS + tid;
+    const float2* p1 = vt + 1 * BP_VTS + tid;
+    const float2* p2 = vt + 2 * BP_VTS + tid;
+    const float2* p3 = vt + 3 * BP_VTS + tid;
+    float2 w0 = p0[0], w1 = p1[0], w2 = p2[0];
+    float ax0 = 0.0f, ay0 = 0.0f, ax1 = 0.0f, ay1 = 0.0f, ax2 = 0.0f, ay2 = 0.0f, ax3 = 0.0f, ay3 = 0.0f;
+    auto tap = [&](float hv, float2 a, float2 b, float2 c_, float2 d) {
+        ax0 = fmaf(a.x, hv, ax0); ay0 = fmaf(a.y, hv, ay0);
+        ax1 = fmaf(b.x, hv, ax1); ay1 = fmaf(b.y, hv, ay1);
+        ax2 = fmaf(c_.x, hv, ax2); ay2 = fmaf(c_.y, hv, ay2);
+        ax3 = fmaf(d.x, hv, ax3); ay3 = fmaf(d.y, hv, ay3);
+    };
+    const int L4 = L & ~3;
+    int i = 0;
+    for (int k = 0; i < L4; ++k, i += 4) {
+        const float4 h4 = *reinterpret_cast<const float4*>(hr + i);
+        const float2 w3 = p3[k], w4 = p0[k + 1], w5 = p1[k + 1], w6 = p2[k + 1];
+        tap(h4.x, w0, w1, w2, w3);
+        tap(h4.y, w1, w2, w3, w4);
+        tap(h4.z, w2, w3, w4, w5);
+        tap(h4.w, w3, w4, w5, w6);
+        w0 = w4; w1 = w5; w2 = w6;
+    }
+    for (; i < L; ++i) {
+        const float2 w3 = v(4 * tid + i + 3);
+        tap(hr[i], w0, w1, w2, w3);
+        w0 = w1; w1 = w2; w2 = w3;
+    }
+    // divided by the number of taps inside the row: i with 0 <= s - half + i < S
+    const float ax[4] = {ax0, ax1, ax2, ax3}, ay[4] = {ay0, ay1, ay2, ay3};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        if (q >= nout) break;
+        const int s = sb + q;
+        const int ia = max(0, half - s), ib = min(L - 1, S - 1 - s + half);
+        const float inv = 1.0f / (float)(ib - ia + 1);
+        orow[s] = make_float2(ax[q] * inv, ay[q] * inv);
     }
 }
 
