@@ -226,6 +226,11 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
     // the tile's 8 depths are the same for every lane: z-only values go to
     // scalar registers (readfirstlane), which frees 16 VGPRs
     auto uni = [](float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); };
+    auto uni_d = [](double v) {
+        const long long b = __double_as_longlong(v);
+        const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffll)), hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+        return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+    };
     // per-voxel constants
     float z2[HT_Z], kz2[HT_Z], base[HT_Z];
     double base_d[HT_Z];  // the same in FP64, anchored per channel block below
@@ -247,6 +252,13 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
         }
         base[j] = (txd + fz) * p.spm + (float)p.delay;
         base_d[j] = ((double)txd + (double)fz) * (double)p.spm + (double)p.delay;
+        // txd = z: depth-only, held in scalar registers (24 VGPRs fewer).  Linear
+        // only: the cubic kernel measured slower so (12.3 against 11.8 ms on the
+        // 128^3 frame), although it then no longer spills.
+        if constexpr (!FOCUS && !CUBIC) {
+            base[j] = uni(base[j]);
+            base_d[j] = uni_d(base_d[j]);
+        }
     }
     // transmit path bounds over the tile
     float txd_lo, txd_hi;
@@ -436,22 +448,8 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
                     if (!chan(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(my_ex), cc)), rho2, apo_c))
                         continue;
                     const char* wbytes = reinterpret_cast<const char*>(win);
-                    // Taps outside the aperture get weight 0, or, with the coherence
-                    // factor, are masked off (and skipped when no lane of the wave is
-                    // inside), which measured faster there (and slower without it).
-                    // chan() already required apo_c > 0.1.
-#pragma unroll
-                    for (int j = 0; j < HT_Z; ++j) {
-                        float w;
-                        if constexpr (SINAPO) {
-                            w = apo_c;
-                        } else if constexpr (CF) {
-                            const float s = rho2 * kz2[j];
-                            if (!(s < HANN_S_THR)) continue;
-                            w = hann_poly(s);
-                        } else {
-                            w = tap_weight<false>(rho2, kz2[j], apo_c);
-                        }
+                    // one tap: sample index, window entry, interpolation, accumulation
+                    auto tap_at = [&](int j, float w) {
                         // the window covers every voxel of the tile with a 2-sample margin,
                         // so the index needs no clamp beyond the reference's [1, S-2], and
                         // none at all in tiles whose every index lies inside it
@@ -472,6 +470,24 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
                         acc[j].x = fmaf(w, v.x, acc[j].x);
                         acc[j].y = fmaf(w, v.y, acc[j].y);
                         if constexpr (CF) inc[j] = fmaf(v.x, v.x, fmaf(v.y, v.y, inc[j]));  // w != 0 here
+                    };
+                    // Taps outside the aperture get weight 0, or, with the coherence
+                    // factor, are masked off (and skipped when no lane of the wave is
+                    // inside), which measured faster there (and slower without it).
+                    // chan() already required apo_c > 0.1.
+#pragma unroll
+                    for (int j = 0; j < HT_Z; ++j) {
+                        float w;
+                        if constexpr (SINAPO) {
+                            w = apo_c;
+                        } else if constexpr (CF) {
+                            const float s = rho2 * kz2[j];
+                            if (!(s < HANN_S_THR)) continue;
+                            w = hann_poly(s);
+                        } else {
+                            w = tap_weight<false>(rho2, kz2[j], apo_c);
+                        }
+                        tap_at(j, w);
                     }
                 }
                 // channels whose window did not fit: direct reads
