@@ -61,6 +61,35 @@ def test_convert_demod_stage(torch_dev, orc, dtype, taps):
     assert api.rf_set_match_filter(None)
 
 
+@pytest.mark.parametrize("S,taps", [(1003, 64), (1003, 0), (1029, 1024), (17, 5), (5, 4), (2, 1)])
+def test_convert_demod_ragged_rows(torch_dev, orc, S, taps):
+    """Row lengths that are not multiples of the four outputs a thread
+    computes, rows shorter than the filter's reach, and the widest filter on a
+    row barely longer than it (nearly every output at a row end)."""
+    import torch
+    print(f"bandpass ragged S={S} taps={taps}", flush=True)
+    T, Cc = 2, 3
+    rng = np.random.default_rng(S * 7 + taps)
+    raw = bandpass_rows(rng, Cc, T, S, np.int16, pad=3)
+    cmap = rng.permutation(Cc).astype(np.int16)
+    R0 = raw.shape[1]
+    assert api.rf_init((R0, Cc), (S, Cc, T), L.NONE)
+    assert api.rf_set_channel_mapping(cmap)
+    h = rng.standard_normal(taps).astype(np.float32) if taps else None
+    assert api.rf_set_match_filter(h)
+    out = torch.full((T * Cc * S * 2,), np.nan, dtype=torch.float32, device=torch_dev)
+    assert api.rf_convert_demod(_t(raw, torch_dev), L.TYPE_I16, out, 5.3e6, 20.8e6)
+    torch.cuda.synchronize()
+    assert api.rf_set_match_filter(None)
+    got = out.cpu().numpy().view(np.complex64).reshape(T, Cc, S)
+    want = orc.convert_demod(raw, L.TYPE_I16, cmap, R0, T, Cc, S, 5.3e6, 20.8e6,
+                             None if h is None else h.astype(np.float64))
+    assert np.all(np.isfinite(got))
+    err = rel_l2(got, want)
+    print(f"  rel L2 {err:.3e}")
+    assert err < 1e-6
+
+
 def _bandpass_frame(acq, seed, taps):
     bp = acq.bp
     S, Cc, T = bp.dec_data_dim[0], bp.dec_data_dim[1], bp.dec_data_dim[2]

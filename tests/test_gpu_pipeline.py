@@ -151,6 +151,44 @@ def test_hilbert_matches_oracle(torch_dev, orc, S, taps):
     assert api.rf_set_match_filter(None)
 
 
+@pytest.mark.parametrize("S,taps", [(4096, 64), (2048, 0), (256, 5), (16, 3)])
+def test_hilbert_odd_row_count(torch_dev, orc, S, taps):
+    """Power-of-two rows are transformed in pairs: with an odd row count the
+    last row pairs with zeros (and, for short rows, the last workgroup holds
+    fewer pairs than it has room for)."""
+    import torch
+    print(f"hilbert odd rows S={S} taps={taps}", flush=True)
+    T, Cc = 3, 5
+    assert api.rf_init((T * S, Cc), (S, Cc, T), L.NONE)  # (no Hadamard matrix of order 3)
+    rng = np.random.default_rng(S + 3 * taps + 1)
+    # rows of very different scale: each pair shares one transform
+    x = (rng.standard_normal((T * Cc, S)) * np.geomspace(1.0, 3000.0, T * Cc)[:, None]).astype(np.float32)
+    h = rng.standard_normal(taps).astype(np.float32) if taps else None
+    assert api.rf_set_match_filter(h)
+    out = torch.full((T * Cc * S * 2,), np.nan, dtype=torch.float32, device=torch_dev)
+    assert api.rf_hilbert(_t(x, torch_dev), 1, out)
+    torch.cuda.synchronize()
+    assert api.rf_set_match_filter(None)
+    got = out.cpu().numpy().view(np.complex64).reshape(T * Cc, S)
+    want = orc.hilbert(x.astype(np.float64), None if h is None else h.astype(np.float64))
+    assert np.all(np.isfinite(got))
+    assert rel_l2(got, want) < 1e-5
+    # every row on its own scale (a row paired with a 3000x larger one included)
+    worst = max(rel_l2(got[r], want[r]) for r in range(T * Cc))
+    print(f"  worst row rel L2 {worst:.2e}")
+    assert worst < 1e-4
+
+
+def test_beamform_decode_none_odd_rows(lib, orc):
+    """decode = NONE takes the fused convert + Hilbert kernel; 5 x 5 rows."""
+    print("beamform decode none, 25 rows", flush=True)
+    acq = configs.rca_small(seed=5, T=5, encoding=L.NONE)
+    acq.bp.decode = L.NONE
+    got = api.beamform_volume(acq.raw, acq.bp)
+    assert got is not None
+    assert rel_l2(got, orc.pipeline(acq)) < 1e-4
+
+
 def _das_case(acq, orc, torch_dev, cubic, tol, cf=False):
     import torch
     st = orc.pipeline(acq, cubic=cubic, return_stages=True)
