@@ -21,6 +21,7 @@ def pairs(torch_dev):
     ps = ml.make_pairs(2, 7, torch_dev, log=None)
     yield ps
     api.set_stream(None)
+    assert api.set_das_options(cubic=True)  # the library default, which later modules expect
 
 
 def run(params, pairs, thr, dev):

@@ -182,6 +182,7 @@ def test_hilbert_odd_row_count(torch_dev, orc, S, taps):
 def test_beamform_decode_none_odd_rows(lib, orc):
     """decode = NONE takes the fused convert + Hilbert kernel; 5 x 5 rows."""
     print("beamform decode none, 25 rows", flush=True)
+    assert api.set_das_options(cubic=True, cf_hercules=False)
     acq = configs.rca_small(seed=5, T=5, encoding=L.NONE)
     acq.bp.decode = L.NONE
     got = api.beamform_volume(acq.raw, acq.bp)
