@@ -46,66 +46,89 @@ def test_header_declarations_are_exported():
             assert m.group(1) in names, (h.name, m.group(1))
 
 
-def _expect_false(acq, mutate, out_elems=None):
+def _expect_false(acq, mutate, out_elems=None, resize_raw=True):
     bp = acq.bp
     mutate(bp)
     n = bp.output_points[0] * bp.output_points[1] * bp.output_points[2]
     out = np.zeros(n if out_elems is None else out_elems, np.complex64)
-    return api.beamform(acq.raw, out, bp)
+    # raw input sized to whatever rf_raw_dim now says, so that only the
+    # parameter under test is wrong (raw_short keeps the original array)
+    raw = np.zeros((bp.rf_raw_dim[1], bp.rf_raw_dim[0]), np.int16) if resize_raw else acq.raw
+    return api.beamform(raw, out, bp)
 
 
-@pytest.mark.parametrize("case", [
-    "sample_type", "data_type", "bad_order", "map_range", "filter_long", "sequence",
-    "herc_focus", "zero_dims", "raw_short", "out_short", "fft_len"])
-def test_beamform_rejects_invalid_parameters(lib, case):
+# Each case must fail for its own reason, checked by the message on stderr:
+# with no GPU every call returns false, so the flag alone would pin nothing.
+REJECTIONS = {
+    "sample_type": (lambda b: setattr(b, "sample_type", L.SAMPLE_BANDPASS), "takes no transmit decoding"),
+    "data_type": (lambda b: setattr(b, "data_type", L.TYPE_U8), "Unsupported input data type"),
+    "bad_order": (lambda b: (b.dec_data_dim.__setitem__(2, 36), b.rf_raw_dim.__setitem__(0, 36 * 1024)),
+                  "Size '36' not supported"),
+    "map_range": (lambda b: b.channel_mapping.__setitem__(3, 999), "entry 3 = 999 outside"),
+    "filter_long": (lambda b: setattr(b, "filter_length", 1025), "Filter length 1025 exceeds 1024"),
+    "sequence": (lambda b: setattr(b, "das_shader_id", L.RCA_VLS), "Unsupported sequence ID 3"),
+    "herc_focus": (lambda b: setattr(b, "tx_orientation", L.ORIENT_NONE), "Invalid focus for HERCULES"),
+    "zero_dims": (lambda b: b.dec_data_dim.__setitem__(0, 0), "Invalid dimensions for RF raw data"),
+    "raw_short": (lambda b: b.rf_raw_dim.__setitem__(1, 64), "rf_raw_dim needs"),
+    "out_short": (lambda b: None, "Output span holds 80 bytes"),
+    "fft_len": (lambda b: (b.dec_data_dim.__setitem__(0, 5000), b.rf_raw_dim.__setitem__(0, 32 * 5000)),
+                "Sample count 5000 is not supported"),
+}
+
+
+@pytest.mark.parametrize("case", sorted(REJECTIONS))
+def test_beamform_rejects_invalid_parameters(lib, capfd, case):
     acq = configs.rca_small(generate=False)
-    mut = {
-        "sample_type": lambda b: setattr(b, "sample_type", L.SAMPLE_BANDPASS),
-        "data_type": lambda b: setattr(b, "data_type", L.TYPE_U8),
-        "bad_order": lambda b: (b.dec_data_dim.__setitem__(2, 36), b.rf_raw_dim.__setitem__(0, 36 * 1024)),
-        "map_range": lambda b: b.channel_mapping.__setitem__(3, 999),
-        "filter_long": lambda b: setattr(b, "filter_length", 1025),
-        "sequence": lambda b: setattr(b, "das_shader_id", L.RCA_VLS),
-        "herc_focus": lambda b: setattr(b, "tx_orientation", L.ORIENT_NONE),
-        "zero_dims": lambda b: b.dec_data_dim.__setitem__(0, 0),
-        "raw_short": lambda b: b.rf_raw_dim.__setitem__(1, 64),
-        "out_short": lambda b: None,
-        "fft_len": lambda b: (b.dec_data_dim.__setitem__(0, 1000), b.rf_raw_dim.__setitem__(0, 32 * 1000)),
-    }[case]
+    mut, message = REJECTIONS[case]
     out_elems = 10 if case == "out_short" else None
-    assert _expect_false(acq, mut, out_elems) is False
+    assert _expect_false(acq, mut, out_elems, resize_raw=case != "raw_short") is False
+    err = capfd.readouterr().err
+    assert message in err, err
 
 
-def test_motion_detection_rejects_invalid_parameters(lib):
+def test_motion_detection_rejects_invalid_parameters(lib, capfd):
     imgs = np.zeros((3, 40, 50), np.float32)
     maps = np.zeros((3, 4, 5, 4), np.float32)
     ok = api.motion_params(50, 40, 3, (5, 4), 8, 8, (4, 4))
     bad = [
-        dict(frame_count=0), dict(reference_frame=3), dict(patch_size=0),
-        dict(search_margins=(100, 100)),  # more offsets than one workgroup holds
+        (dict(frame_count=0), "empty images"),
+        (dict(reference_frame=3), "reference frame out of range"),
+        (dict(patch_size=0), "empty motion grid or patch"),
     ]
-    for b in bad:
+    for b, message in bad:
         p = api.motion_params(50, 40, 3, (5, 4), 8, 8, (4, 4))
         for k, v in b.items():
-            if isinstance(v, tuple):
-                getattr(p, k)[:] = v
-            else:
-                setattr(p, k, v)
+            setattr(p, k, v)
         assert api.motion_detection(imgs, maps, p) is False, b
+        assert message in capfd.readouterr().err, b
     assert api.motion_detection(imgs[:1], maps, ok) is False           # image span too small
+    assert "image span holds 8000 bytes" in capfd.readouterr().err
     assert api.motion_detection(imgs, maps[:1], ok) is False           # map span too small
+    assert "motion map span holds 320 bytes" in capfd.readouterr().err
+    # more offsets than one workgroup's map holds (the search is clamped to
+    # the image, so the image must be larger than the window for this)
+    big = np.zeros((2, 200, 200), np.float32)
+    p = api.motion_params(200, 200, 2, (5, 4), 8, 8, (100, 100))
+    assert api.motion_detection(big, np.zeros((2, 4, 5, 4), np.float32), p) is False
+    assert "(2*100+1) x (2*100+1) offsets exceeds" in capfd.readouterr().err
 
 
-def test_corr_images_rejects_invalid_dims(lib):
+def test_corr_images_rejects_invalid_dims(lib, capfd):
     t = np.zeros((5, 5), np.float32)
     s = np.zeros((4, 9), np.float32)
     assert api.corr_images(t, s, np.zeros((1, 5), np.float32)) is False  # template taller than source
+    assert "Template dimensions must be <= source dimensions" in capfd.readouterr().err
     assert api.corr_images(np.zeros((0, 3), np.float32), s, np.zeros((1, 1), np.float32)) is False
+    assert "Invalid image dimensions" in capfd.readouterr().err
     assert api.corr_images(np.zeros((2, 3), np.float32), s, np.zeros((1, 1), np.float32)) is False  # map too small
+    assert "corr_map span is too small" in capfd.readouterr().err
 
 
-def test_c_api_call_order_errors(lib):
+def test_c_api_call_order_errors(lib, capfd):
     # decode / hilbert before buffers are registered
     assert api.cuda_decode(0, 0) is False
+    assert "OGL buffers not registered" in capfd.readouterr().err
     assert api.cuda_hilbert(0, 1) is False
+    assert "OGL buffers not registered" in capfd.readouterr().err
     assert api.register_cuda_buffers(list(range(1, 18)), 99) is False  # > 16 buffers
+    assert "Too many RF data buffers" in capfd.readouterr().err
