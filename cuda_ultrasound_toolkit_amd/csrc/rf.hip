@@ -92,14 +92,23 @@ __global__ __launch_bounds__(256) void convert_demod_kernel(const T* __restrict_
     const T* r = raw + (size_t)(uint16_t)map[c] * R0 + 2 * (size_t)t * S;
     // every thread issues all its loads first
     constexpr int FILL = (BP_SEG + MAX_FILTER_LENGTH) / 256;
+    struct alignas(2 * sizeof(T)) IQPair { T i, q; };
+    // (I, Q) as one load when the row start allows it (uniform per workgroup)
+    const bool paired = (reinterpret_cast<uintptr_t>(r) & (2 * sizeof(T) - 1)) == 0;
     float Iv[FILL], Qv[FILL];
 #pragma unroll
     for (int u = 0; u < FILL; ++u) {
         const int js = org + tid + 256 * u;  // sample index in the row
         Iv[u] = Qv[u] = 0.0f;
         if (tid + 256 * u < nwin && js >= 0 && js < S) {
-            Iv[u] = static_cast<float>(r[2 * (size_t)js]);
-            Qv[u] = static_cast<float>(r[2 * (size_t)js + 1]);
+            if (paired) {
+                const IQPair w = reinterpret_cast<const IQPair*>(r)[js];
+                Iv[u] = static_cast<float>(w.i);
+                Qv[u] = static_cast<float>(w.q);
+            } else {
+                Iv[u] = static_cast<float>(r[2 * (size_t)js]);
+                Qv[u] = static_cast<float>(r[2 * (size_t)js + 1]);
+            }
         }
     }
 #pragma unroll
@@ -158,13 +167,22 @@ __global__ __launch_bounds__(256) void convert_demod_kernel(const T* __restrict_
     }
     // divided by the number of taps inside the row: i with 0 <= s - half + i < S
     const float ax[4] = {ax0, ax1, ax2, ax3}, ay[4] = {ay0, ay1, ay2, ay3};
+    float2 y[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        if (q >= nout) break;
         const int s = sb + q;
         const int ia = max(0, half - s), ib = min(L - 1, S - 1 - s + half);
-        const float inv = 1.0f / (float)(ib - ia + 1);
-        orow[s] = make_float2(ax[q] * inv, ay[q] * inv);
+        const float inv = 1.0f / (float)max(1, ib - ia + 1);
+        y[q] = make_float2(ax[q] * inv, ay[q] * inv);
+    }
+    if (nout == 4 && (reinterpret_cast<uintptr_t>(orow + sb) & 15) == 0) {
+        float4* o4 = reinterpret_cast<float4*>(orow + sb);
+        o4[0] = make_float4(y[0].x, y[0].y, y[1].x, y[1].y);
+        o4[1] = make_float4(y[2].x, y[2].y, y[3].x, y[3].y);
+    } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (q < nout) orow[sb + q] = y[q];
     }
 }
 
