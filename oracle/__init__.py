@@ -67,6 +67,9 @@ def load() -> C.CDLL:
     lib.or_das_constants.argtypes = [C.c_void_p, C.POINTER(DasConsts)]
     lib.or_das.restype = C.c_int
     lib.or_das.argtypes = [DP, C.POINTER(DasConsts), C.c_int, C.c_int, DP, C.POINTER(C.c_longlong)]
+    lib.or_das_planes.restype = C.c_int
+    lib.or_das_planes.argtypes = [DP, C.POINTER(DasConsts), C.c_int, C.c_int, C.POINTER(C.c_int), C.c_int, DP,
+                                  C.POINTER(C.c_longlong)]
     lib.or_zncc_valid.argtypes = [DP, C.c_int, C.c_int, C.c_int, DP, C.c_int, C.c_int, C.c_int, DP]
     lib.or_motion_detection.restype = C.c_int
     lib.or_motion_detection.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.POINTER(C.c_float)]
@@ -172,21 +175,30 @@ def das_constants(bp) -> DasConsts:
     return k
 
 
-def das(iq: np.ndarray, bp, cubic: bool = True, cf_hercules: bool = False):
-    """iq: complex [T][C][S] -> (complex volume [Nz][Ny][Nx], active tap count)."""
+def das(iq: np.ndarray, bp, cubic: bool = True, cf_hercules: bool = False, planes=None):
+    """iq: complex [T][C][S] -> (complex volume [Nz][Ny][Nx], active tap count).
+    planes: depth-plane indices of the grid to evaluate (volume [len(planes)][Ny][Nx])."""
     k = das_constants(bp)
     iqd = np.ascontiguousarray(np.stack([iq.real, iq.imag], axis=-1), np.float64)
-    out = np.zeros(2 * k.Nx * k.Ny * k.Nz)
+    nz = k.Nz if planes is None else len(planes)
+    out = np.zeros(2 * k.Nx * k.Ny * nz)
     act = C.c_longlong(0)
-    rc = load().or_das(_dp(iqd), C.byref(k), 0 if cubic else 1, 1 if cf_hercules else 0, _dp(out), C.byref(act))
+    if planes is None:
+        rc = load().or_das(_dp(iqd), C.byref(k), 0 if cubic else 1, 1 if cf_hercules else 0, _dp(out),
+                           C.byref(act))
+    else:
+        pl = np.ascontiguousarray(planes, np.int32)
+        rc = load().or_das_planes(_dp(iqd), C.byref(k), 0 if cubic else 1, 1 if cf_hercules else 0,
+                                  pl.ctypes.data_as(C.POINTER(C.c_int)), len(pl), _dp(out), C.byref(act))
     if rc != 0:
         raise ValueError("oracle DAS rejected the parameters")
-    vol = (out[0::2] + 1j * out[1::2]).reshape(k.Nz, k.Ny, k.Nx)
+    vol = (out[0::2] + 1j * out[1::2]).reshape(nz, k.Ny, k.Nx)
     return vol, act.value
 
 
-def pipeline(acq, cubic: bool = True, cf_hercules: bool = False, return_stages: bool = False):
-    """The whole reference chain on the CPU: convert -> decode -> Hilbert -> DAS."""
+def pipeline(acq, cubic: bool = True, cf_hercules: bool = False, return_stages: bool = False, planes=None):
+    """The whole reference chain on the CPU: convert -> decode -> Hilbert -> DAS
+    (on the depth planes `planes` of the grid only, when given)."""
     bp = acq.bp
     S, Cc, T = bp.dec_data_dim[0], bp.dec_data_dim[1], bp.dec_data_dim[2]
     R0 = bp.rf_raw_dim[0]
@@ -194,7 +206,7 @@ def pipeline(acq, cubic: bool = True, cf_hercules: bool = False, return_stages: 
         h = np.array(bp.rf_filter[: bp.filter_length], np.float64) if bp.filter_length else None
         iq = convert_demod(acq.raw, bp.data_type, np.array(bp.channel_mapping[:Cc], np.int16), R0, T, Cc, S,
                            bp.center_frequency, bp.sampling_frequency, h)
-        vol, act = das(iq, bp, cubic, cf_hercules)
+        vol, act = das(iq, bp, cubic, cf_hercules, planes)
         if return_stages:
             return dict(iq=iq, vol=vol, active=act)
         return vol
@@ -206,7 +218,7 @@ def pipeline(acq, cubic: bool = True, cf_hercules: bool = False, return_stages: 
         dec = conv
     h = np.array(bp.rf_filter[: bp.filter_length], np.float64) if bp.filter_length else None
     iq = hilbert(dec.reshape(T * Cc, S), h).reshape(T, Cc, S)
-    vol, act = das(iq, bp, cubic, cf_hercules)
+    vol, act = das(iq, bp, cubic, cf_hercules, planes)
     if return_stages:
         return dict(conv=conv, dec=dec, iq=iq, vol=vol, active=act)
     return vol

@@ -577,6 +577,12 @@ static void tpw_voxel(const double* iq, const or_das_consts* k, double vx, doubl
 /* HERCULES: templated_beamformer.cuh:102-216; FORCES: :219-324. */
 int or_das(const double* iq, const or_das_consts* k, int interp, int cf_mode, double* out, long long* active)
 {
+    return or_das_planes(iq, k, interp, cf_mode, NULL, k->Nz, out, active);
+}
+
+int or_das_planes(const double* iq, const or_das_consts* k, int interp, int cf_mode, const int* planes, int n_planes,
+                  double* out, long long* active)
+{
     const int S = k->S, C = k->C, T = k->T;
     const int G = k->readi_groups;
     uint64_t row_bits = 0;
@@ -589,13 +595,20 @@ int or_das(const double* iq, const or_das_consts* k, int interp, int cf_mode, do
     if (legacy && (T > MAX_ACQ_COUNT || k->focus_type == 0)) return -1;
     if (k->sequence == RCA_TPW && k->focus_type != 1 && k->focus_type != 2) return -1;
     if (k->sequence == UFORCES && k->tx_element[0] == -1) return -1;
-    const long long nvox = (long long)k->Nx * k->Ny * k->Nz;
+    if (planes) {
+        for (int q = 0; q < n_planes; ++q)
+            if (planes[q] < 0 || planes[q] >= k->Nz) return -1;
+    } else if (n_planes != k->Nz) {
+        return -1;
+    }
+    const long long nvox = (long long)k->Nx * k->Ny * n_planes;
     long long act_total = 0;
 #pragma omp parallel for schedule(dynamic, 64) reduction(+ : act_total)
     for (long long v = 0; v < nvox; ++v) {
         int ix = (int)(v % k->Nx);
         int iy = (int)((v / k->Nx) % k->Ny);
-        int iz = (int)(v / ((long long)k->Nx * k->Ny));
+        int pq = (int)(v / ((long long)k->Nx * k->Ny));
+        int iz = planes ? planes[pq] : pq;
         double vx = (double)k->min_x + (double)ix * (double)k->res_x;
         double vy = (double)k->min_y + (double)iy * (double)k->res_y;
         double vz = (double)k->min_z + (double)iz * (double)k->res_z;

@@ -69,6 +69,10 @@ int or_das_constants(const CudaBeamformerParameters* bp, or_das_consts* k);
  * with apo above the sequence's threshold through *active (may be NULL).  out:
  * interleaved complex [Nz][Ny][Nx]. */
 int or_das(const double* iq, const or_das_consts* k, int interp, int cf_mode, double* out, long long* active);
+/* The same on the depth planes planes[0..n_planes) of the grid only (NULL with
+ * n_planes = Nz: every plane); out: interleaved complex [n_planes][Ny][Nx]. */
+int or_das_planes(const double* iq, const or_das_consts* k, int interp, int cf_mode, const int* planes, int n_planes,
+                  double* out, long long* active);
 int or_readi_row_bits(const or_das_consts* k, uint64_t* bits);
 
 /* ---- 2-D block matching (image_processor.cpp:106-325, block_match_kernels.cu) ---- */
