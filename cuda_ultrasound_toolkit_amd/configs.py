@@ -182,9 +182,18 @@ def uneven_frames(seed: int = 4, n: int = 64, small: bool = False, generate: boo
     The imaged depth follows the sample count (deepest point 90 % of the
     round-trip range).  small: a reduced set for tests (transmits {16, 32},
     32 channels, samples U{600..1100}, sides U{12..40}, 20-60 scatterers)."""
+    return [_uneven(seed, i, d, generate) for i, d in enumerate(_uneven_draws(seed, n, small))]
+
+
+def uneven_frame(seed: int, index: int, small: bool = False, generate: bool = True) -> Acquisition:
+    """Frame `index` of uneven_frames(seed, ...), alone (the same sizes and data)."""
+    return _uneven(seed, index, _uneven_draws(seed, index + 1, small)[index], generate)
+
+
+def _uneven_draws(seed: int, n: int, small: bool) -> list:
     rng = np.random.default_rng(seed)
     out = []
-    for i in range(n):
+    for _ in range(n):
         if small:
             T = int(rng.choice([16, 32]))
             Cc, S = 32, int(rng.integers(600, 1101))
@@ -195,12 +204,17 @@ def uneven_frames(seed: int = 4, n: int = 64, small: bool = False, generate: boo
             Cc, S = 128, int(rng.integers(1024, 4097))
             nsc, side = int(rng.integers(50, 501)), int(rng.integers(64, 161))
             taps = 64
-        half = 0.5 * Cc * 0.3e-3
-        zmax = min(60e-3, 0.9 * S * 1540.0 / (2 * 20e6))
-        zmin = min(5e-3, 0.25 * zmax)
-        out.append(rca_3d(seed=seed * 1000 + i, T=T, C=Cc, S=S, n=side, n_scatterers=nsc, filter_taps=taps,
-                          z_range=(zmin, zmax), half_width=half, box_half=half, generate=generate))
+        out.append((T, Cc, S, nsc, side, taps))
     return out
+
+
+def _uneven(seed: int, i: int, draw, generate: bool) -> Acquisition:
+    T, Cc, S, nsc, side, taps = draw
+    half = 0.5 * Cc * 0.3e-3
+    zmax = min(60e-3, 0.9 * S * 1540.0 / (2 * 20e6))
+    zmin = min(5e-3, 0.25 * zmax)
+    return rca_3d(seed=seed * 1000 + i, T=T, C=Cc, S=S, n=side, n_scatterers=nsc, filter_taps=taps,
+                  z_range=(zmin, zmax), half_width=half, box_half=half, generate=generate)
 
 
 def rca_small(seed: int = 3, **kw) -> Acquisition:

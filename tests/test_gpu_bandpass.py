@@ -61,16 +61,20 @@ def test_convert_demod_stage(torch_dev, orc, dtype, taps):
     assert api.rf_set_match_filter(None)
 
 
+@pytest.mark.parametrize("dtype", [L.TYPE_I16, L.TYPE_F32])
 @pytest.mark.parametrize("S,taps", [(1003, 64), (1003, 0), (1029, 1024), (17, 5), (5, 4), (2, 1)])
-def test_convert_demod_ragged_rows(torch_dev, orc, S, taps):
+def test_convert_demod_ragged_rows(torch_dev, orc, S, taps, dtype):
     """Row lengths that are not multiples of the four outputs a thread
     computes, rows shorter than the filter's reach, and the widest filter on a
-    row barely longer than it (nearly every output at a row end)."""
+    row barely longer than it (nearly every output at a row end).  A padding
+    of 3 samples makes rf_raw_dim[0] odd, so half of the rows start off the
+    (I, Q) pair alignment (8 bytes for float32) and take the unpaired loads."""
     import torch
-    print(f"bandpass ragged S={S} taps={taps}", flush=True)
+    print(f"bandpass ragged S={S} taps={taps} dtype={dtype}", flush=True)
     T, Cc = 2, 3
     rng = np.random.default_rng(S * 7 + taps)
-    raw = bandpass_rows(rng, Cc, T, S, np.int16, pad=3)
+    raw = bandpass_rows(rng, Cc, T, S, np.int16 if dtype == L.TYPE_I16 else np.float32, pad=3)
+    assert raw.shape[1] % 2 == 1
     cmap = rng.permutation(Cc).astype(np.int16)
     R0 = raw.shape[1]
     assert api.rf_init((R0, Cc), (S, Cc, T), L.NONE)
@@ -78,11 +82,11 @@ def test_convert_demod_ragged_rows(torch_dev, orc, S, taps):
     h = rng.standard_normal(taps).astype(np.float32) if taps else None
     assert api.rf_set_match_filter(h)
     out = torch.full((T * Cc * S * 2,), np.nan, dtype=torch.float32, device=torch_dev)
-    assert api.rf_convert_demod(_t(raw, torch_dev), L.TYPE_I16, out, 5.3e6, 20.8e6)
+    assert api.rf_convert_demod(_t(raw, torch_dev), dtype, out, 5.3e6, 20.8e6)
     torch.cuda.synchronize()
     assert api.rf_set_match_filter(None)
     got = out.cpu().numpy().view(np.complex64).reshape(T, Cc, S)
-    want = orc.convert_demod(raw, L.TYPE_I16, cmap, R0, T, Cc, S, 5.3e6, 20.8e6,
+    want = orc.convert_demod(raw, dtype, cmap, R0, T, Cc, S, 5.3e6, 20.8e6,
                              None if h is None else h.astype(np.float64))
     assert np.all(np.isfinite(got))
     err = rel_l2(got, want)

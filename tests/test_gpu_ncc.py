@@ -33,7 +33,7 @@ def compare(got, want):
     assert np.allclose(got[..., :2], want[..., :2], atol=2e-3), np.abs(got[..., :2] - want[..., :2]).max()
 
 
-@pytest.mark.parametrize("case", ["basic", "edges", "nosub", "neighbour", "rect", "noisy"])
+@pytest.mark.parametrize("case", ["basic", "edges", "nosub", "neighbour", "rect", "noisy", "wide"])
 def test_motion_detection_matches_oracle(lib, orc, case):
     print(f"motion {case}", flush=True)
     if case == "basic":
@@ -51,6 +51,12 @@ def test_motion_detection_matches_oracle(lib, orc, case):
     elif case == "rect":  # different x / y margins, large search
         imgs = frames_with_shifts((100, 90), [(0, 0), (-7, 3)], 5)
         p = api.motion_params(90, 100, 2, (5, 6), 16, 18, (9, 5), abs_thr=0.3, rel_thr=0.0)
+    elif case == "wide":
+        # (2*24+1)^2 = 2401 offsets: the peak search reads the first 2048 map
+        # entries only (block_match_kernels.cu:7-64); the third frame's shift
+        # (dy = +19) lies in the truncated rows
+        imgs = frames_with_shifts((128, 136), [(0, 0), (3, -2), (5, 19)], 9)
+        p = api.motion_params(136, 128, 3, (4, 4), 24, 16, (24, 24), abs_thr=0.3, rel_thr=0.0)
     else:
         imgs = frames_with_shifts((72, 72), [(0, 0), (2, -3)], 6, noise=0.2)
         p = api.motion_params(72, 72, 2, (5, 5), 14, 16, (5, 5), abs_thr=0.2, rel_thr=0.01, min_var=0.01)

@@ -392,3 +392,57 @@ def test_full_size_rca3d_depth_planes(torch_dev, orc, cubic):
     err = rel_l2(got, want)
     print(f"  rel L2 {err:.3e}; max |err| / max |v| {np.abs(got - want).max() / np.abs(want).max():.3e}")
     assert err < 1e-4
+
+
+@pytest.mark.parametrize("T,S", [(32, 512), (128, 256), (24, 200)])
+def test_decode_f32_input(torch_dev, orc, T, S):
+    """TYPE_F32 raw input with Hadamard decode (the FP32 MFMA GEMM path; the
+    reference's cuBLAS SGEMM, hadamard_decoder.cpp:49-78).  Integer-valued
+    samples decode bit-exactly for T = 2^n (every partial sum is an integer
+    below 2^24); arbitrary floats stay within the error bound of any FP32
+    summation order: |got - exact| <= u * sum_k |x_k| + u * |exact|, u = 2^-24
+    (the T-term sum errs by at most (T-1) u sum |x_k|, 1/T scales it down)."""
+    import torch
+    print(f"decode f32 input T={T} S={S}", flush=True)
+    acq = configs.linear_2d(T=T, S=S, seed=T * 3 + S)
+    S_, Cc, T_ = _rf_setup(acq)
+    H = orc.hadamard(T, L.HADAMARD)
+    pow2 = (T & (T - 1)) == 0
+    rng = np.random.default_rng(T + S)
+    for kind in ("integer", "fraction"):
+        raw = acq.raw.astype(np.float32)
+        if kind == "fraction":
+            raw = (raw * np.float32(0.731) + rng.standard_normal(raw.shape).astype(np.float32) * 37.0).astype(np.float32)
+        conv = orc.convert(raw, L.TYPE_F32, acq.channel_map, T * S, T, Cc, S)
+        want = orc.decode(conv, H)
+        out = torch.full((T * Cc * S,), np.nan, dtype=torch.float32, device=torch_dev)
+        assert api.rf_convert_decode(_t(raw, torch_dev), L.TYPE_F32, out, 1, True)
+        torch.cuda.synchronize()
+        got = out.cpu().numpy().reshape(T, Cc, S).astype(np.float64)
+        if kind == "integer" and pow2:
+            assert np.array_equal(got, want.astype(np.float32).astype(np.float64))
+            continue
+        u = 2.0 ** -24
+        bound = u * np.einsum("jk,kcs->jcs", np.abs(H), np.abs(conv)) + u * np.abs(want)
+        if not pow2:  # one more rounding: sum * fl(1/T)
+            bound += u * np.abs(want)
+        excess = np.abs(got - want) / bound
+        print(f"  {kind}: worst error / bound {excess.max():.3f}")
+        assert np.all(np.abs(got - want) <= bound)
+
+
+def test_beamform_f32_matlab_shim(lib, orc):
+    """beamform_f32_matlab (cuda_toolkit_matlab.cpp:24-37): float32 raw frame
+    with Hadamard decode, the whole path against the oracle."""
+    import ctypes as C
+    print("beamform f32 matlab shim", flush=True)
+    for make in (lambda: configs.linear_2d(seed=45), lambda: configs.rca_small(seed=46)):
+        acq = make()
+        acq.raw = (acq.raw.astype(np.float32) * np.float32(0.37)).astype(np.float32)
+        acq.bp.data_type = L.TYPE_F32
+        out = np.zeros(acq.volume_shape, np.complex64)
+        lib.beamform_f32_matlab(acq.raw.ctypes.data_as(C.POINTER(C.c_float)), acq.bp,
+                                out.ctypes.data_as(C.POINTER(C.c_float)))
+        err = rel_l2(out, orc.pipeline(acq))
+        print(f"  rel L2 {err:.3e}")
+        assert err < 1e-4
