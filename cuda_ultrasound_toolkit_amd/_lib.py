@@ -187,6 +187,7 @@ def load() -> C.CDLL:
     _proto(lib, "cut_rf_convert_demod", B, [V, C.c_int, V, C.c_float, C.c_float])
     _proto(lib, "cut_rf_hilbert", B, [V, C.c_int, V])
     _proto(lib, "cut_das_device", B, [V, V, P(CudaBeamformerParameters)])
+    _proto(lib, "cut_das_device_noalloc", B, [V, V, P(CudaBeamformerParameters)])
     _proto(lib, "cut_rf_hilbert_format", B, [V, C.c_int, V, C.c_int, P(C.c_float)])
     _proto(lib, "cut_das_device_format", B, [V, C.c_int, C.c_float, V, P(CudaBeamformerParameters)])
     _proto(lib, "cut_das_active_taps", B, [P(CudaBeamformerParameters), P(C.c_ulonglong)])
@@ -219,6 +220,7 @@ def exported_symbols() -> list[str]:
         "cut_set_stream", "cut_synchronize", "cut_version", "cut_set_decode_mode", "cut_set_das_options",
         "cut_beamform_device", "cut_rf_init", "cut_rf_set_channel_mapping", "cut_rf_set_match_filter",
         "cut_rf_convert_decode", "cut_rf_convert_demod", "cut_rf_hilbert", "cut_das_device", "cut_das_active_taps", "cut_gl_register_device_buffers",
+        "cut_das_device_noalloc",
         "cut_ncc3d_search", "cut_envelope", "cut_rf_hilbert_format", "cut_das_device_format",
         "cut_beamform_batch", "cut_set_autotune", "cut_batch_launch_shape",
         "cut_pnet_param_count", "cut_pnet_set_precision", "cut_pnet_set_input_mode", "cut_pnet_gather", "cut_pnet_forward", "cut_pnet_loss_grad", "cut_pnet_adam",

@@ -178,6 +178,11 @@ def das_device(d_iq, d_volume, bp: L.CudaBeamformerParameters) -> bool:
     return bool(L.load().cut_das_device(d_iq.data_ptr(), d_volume.data_ptr(), C.byref(bp)))
 
 
+def das_device_noalloc(d_iq, d_volume, bp: L.CudaBeamformerParameters) -> bool:
+    """cut_das_device with its workspace in the library's fixed device array (allocates nothing)."""
+    return bool(L.load().cut_das_device_noalloc(d_iq.data_ptr(), d_volume.data_ptr(), C.byref(bp)))
+
+
 def _ptr(t):
     return None if t is None else t.data_ptr()
 

@@ -165,8 +165,11 @@ void set_das_autotune(bool on);
 size_t das_workspace_bytes(const DasParams& p);
 // iq_fmt: an IqFormat; the stored samples are value * iq_scale.  Staged
 // (F16/BF16) IQ is supported by the HERCULES kernel with Hann apodization.
+// The HERCULES transmit split is limited to what the workspace holds.
+// allow_tune false: no launch-shape timing (and no HIP events created) on
+// this call; a shape tuned earlier for the same key is still used.
 bool launch_das(const void* iq, int iq_fmt, float iq_scale, const DasParams& p, const DasOptions& o,
-                float2* vol, void* workspace, size_t workspace_bytes, hipStream_t st);
+                float2* vol, void* workspace, size_t workspace_bytes, hipStream_t st, bool allow_tune = true);
 
 // Batched HERCULES DAS: frames of any sizes in one launch.  Items are
 // (frame, tile) pairs, tile = wz * nxt * nyt + wy * nxt + wx with wz counted

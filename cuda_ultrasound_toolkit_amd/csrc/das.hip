@@ -1038,9 +1038,11 @@ int vox_waves(long long nvox)
 // the default once untimed (first-launch costs), then times every candidate
 // (1, 2, 4, ... up to max_cand) twice and keeps the fastest.  Every candidate
 // computes the complete result (only the summation order differs), so the
-// last timed run's output stands.
+// last timed run's output stands.  allow_tune false: never time (no events are
+// created), use the tuned choice if there is one, else the default.
 template <class Run>
-bool tuned_run(std::map<TuneKey, int>& table, const TuneKey& key, int dflt, int max_cand, hipStream_t st, Run&& run)
+bool tuned_run(std::map<TuneKey, int>& table, const TuneKey& key, int dflt, int max_cand, hipStream_t st, bool allow_tune,
+               Run&& run)
 {
     int choice = dflt;
     bool tune = false;
@@ -1048,7 +1050,7 @@ bool tuned_run(std::map<TuneKey, int>& table, const TuneKey& key, int dflt, int 
         std::lock_guard<std::mutex> lock(g_tune_mutex);
         auto it = table.find(key);
         if (it != table.end()) choice = it->second;
-        else tune = g_das_autotune && max_cand > 1;
+        else tune = allow_tune && g_das_autotune && max_cand > 1;
     }
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     if (!tune || hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return run(choice);
@@ -1227,7 +1229,7 @@ bool launch_das_batch(const BatchFrameDev* d_frames, const int2* d_items, int n_
 }
 
 bool launch_das(const void* iq, int iq_fmt, float iq_scale, const DasParams& p, const DasOptions& o, float2* vol,
-                void* workspace, size_t workspace_bytes, hipStream_t st)
+                void* workspace, size_t workspace_bytes, hipStream_t st, bool allow_tune)
 {
     if (iq_fmt != IQ_F32 && iq_fmt != IQ_F16 && iq_fmt != IQ_BF16) CUT_FAIL("DAS: unknown IQ format " << iq_fmt);
     if (!(iq_scale > 0.0f)) CUT_FAIL("DAS: IQ scale must be positive");
@@ -1250,8 +1252,9 @@ bool launch_das(const void* iq, int iq_fmt, float iq_scale, const DasParams& p, 
             return sp <= 1 || (workspace && workspace_bytes >= (size_t)sp * nvox * (cf ? sizeof(float4) : sizeof(float2)));
         };
         auto run = [&](int splits) -> bool {
-            // a choice tuned with a larger workspace falls back to one that fits
-            if (!fits(splits)) splits = fits(dflt) ? dflt : 1;
+            // a choice tuned with a larger workspace falls back to the largest
+            // power-of-two split that fits
+            while (splits > 1 && !fits(splits)) splits >>= 1;
             hg.splits = splits;
             hg.gt_per_split = (G * p.T + splits - 1) / splits;
             void* part = splits > 1 ? workspace : nullptr;
@@ -1271,7 +1274,7 @@ bool launch_das(const void* iq, int iq_fmt, float iq_scale, const DasParams& p, 
         };
         const TuneKey key{p.Nx, p.Ny, p.Nz, p.T, p.C, p.S, G, iq_fmt, o.cubic, cf, focus, sinapo};
         const int cand = fits(max_splits(G * p.T)) ? max_splits(G * p.T) : 1;
-        if (!tuned_run(g_tuned_splits, key, dflt, cand, st, run)) return false;
+        if (!tuned_run(g_tuned_splits, key, dflt, cand, st, allow_tune, run)) return false;
     } else if (p.sequence == FORCES) {
         if (p.readi_groups > 1 && p.encoding != HADAMARD) CUT_FAIL("FORCES only supports Hadamard encoding at this time.");
         if (iq_fmt != IQ_F32) CUT_FAIL("DAS: staged IQ formats are supported for HERCULES only");
@@ -1286,7 +1289,7 @@ bool launch_das(const void* iq, int iq_fmt, float iq_scale, const DasParams& p, 
             return true;
         };
         const TuneKey key{FORCES, p.Nx, p.Ny, p.Nz, p.T, p.C, p.S, p.readi_groups, o.cubic, p.apo_type, 0, 0};
-        if (!tuned_run(g_tuned_waves, key, vox_waves(nvox), VOX_MAX_WAVES, st, run)) return false;
+        if (!tuned_run(g_tuned_waves, key, vox_waves(nvox), VOX_MAX_WAVES, st, allow_tune, run)) return false;
     } else if (p.sequence == UFORCES || p.sequence == RCA_TPW) {
         if (iq_fmt != IQ_F32) CUT_FAIL("DAS: staged IQ formats are supported for HERCULES only");
         if (p.T > MAX_ACQ_COUNT) CUT_FAIL("DAS: at most " << MAX_ACQ_COUNT << " transmits");
@@ -1319,7 +1322,7 @@ bool launch_das(const void* iq, int iq_fmt, float iq_scale, const DasParams& p, 
             return true;
         };
         const TuneKey key{p.sequence, p.Nx, p.Ny, p.Nz, p.T, p.C, p.S, p.focus_type, o.cubic, 0, 0, 0};
-        if (!tuned_run(g_tuned_waves, key, vox_waves(nvox), VOX_MAX_WAVES, st, run)) return false;
+        if (!tuned_run(g_tuned_waves, key, vox_waves(nvox), VOX_MAX_WAVES, st, allow_tune, run)) return false;
     } else {
         CUT_FAIL("Beamformer: Unsupported sequence ID " << p.sequence);
     }

@@ -107,6 +107,16 @@ CUT_EXT_EXPORT bool cut_rf_hilbert(const float* d_in, int in_stride, void* d_out
 /* DAS of complex IQ [T][C][S] into a complex volume [Nz][Ny][Nx]
  * (cuda_toolkit/src/beamformer/beamformer.cu:139-197). */
 CUT_EXT_EXPORT bool cut_das_device(const void* d_iq, void* d_volume, const CudaBeamformerParameters* bp);
+/* cut_das_device without any allocation (no counterpart in the reference):
+ * for callers that cannot allocate device memory, such as code under HIP
+ * graph capture or a tight serving loop.  The workspace (transmit-split
+ * partial sums, RCA_TPW angle table) is a fixed 128 MiB device array per GPU,
+ * held until the process ends; the HERCULES transmit split is limited to what
+ * it holds (8 splits at 128^3 without the coherence factor, 4 with it), and
+ * larger volumes run unsplit.  No launch shape is tuned by this call.  Calls
+ * that overlap on one GPU (enqueued on two streams at once) race on the
+ * array. */
+CUT_EXT_EXPORT bool cut_das_device_noalloc(const void* d_iq, void* d_volume, const CudaBeamformerParameters* bp);
 /* The same two stages with IQ in a given storage format (see
  * cut_set_das_options): the Hilbert output is written as value * scale
  * (*scale_out receives the scale used: 2^-k for FP16, 1 otherwise) and the
