@@ -210,8 +210,10 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
     const int ix = bx + lx, iy = by + ly;
     const bool col_valid = ix < p.Nx && iy < p.Ny;
 
-    const float vx = p.min_x + (float)min(ix, p.Nx - 1) * p.res_x;
-    const float vy = p.min_y + (float)min(iy, p.Ny - 1) * p.res_y;
+    // voxel coordinates rounded once from FP64 (as the depths below)
+    const double vy_d = (double)p.min_y + (double)min(iy, p.Ny - 1) * (double)p.res_y;
+    const float vx = (float)((double)p.min_x + (double)min(ix, p.Nx - 1) * (double)p.res_x);
+    const float vy = (float)vy_d;
 
     // tile bounding box
     const float xa = p.min_x + (float)bx * p.res_x, xb = p.min_x + (float)min(bx + HT_X - 1, p.Nx - 1) * p.res_x;
@@ -231,7 +233,10 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
         const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffll)), hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
         return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
     };
-    // per-voxel constants
+    // per-voxel constants.  The depth is formed in FP64 (min_z + iz res_z from
+    // the float grid constants): a float depth is off by up to half an ulp of
+    // ~0.06 m, the same error for every tap of a plane (2e-5 samples on both
+    // paths), which does not average out over the taps as rounding does.
     float z2[HT_Z], kz2[HT_Z], base[HT_Z];
     double base_d[HT_Z];  // the same in FP64, anchored per channel block below
     bool zvalid[HT_Z];
@@ -239,19 +244,20 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
     for (int j = 0; j < HT_Z; ++j) {
         const int iz = bz + j;
         zvalid[j] = iz < p.Nz;
-        const float z = p.min_z + (float)min(iz, p.Nz - 1) * p.res_z;
-        z2[j] = uni(z * z);
+        const double zd = (double)p.min_z + (double)min(iz, p.Nz - 1) * (double)p.res_z;
+        const float z = (float)zd;
+        z2[j] = uni((float)(zd * zd));
         const float kz = p.fn_rx / z * 0.5f;
         kz2[j] = uni(kz * kz);
-        float txd;
+        double txd;
         if constexpr (FOCUS) {
-            const float ty = vy, tzz = z - fz;
-            txd = copysignf(sqrtf(ty * ty + tzz * tzz), tzz) + fz;
+            const double ty = vy_d, tzz = zd - (double)fz;
+            txd = copysign(sqrt(ty * ty + tzz * tzz), tzz) + (double)fz;
         } else {
-            txd = z;  // (0, y sin 0, z cos 0): steering angle zero
+            txd = zd;  // (0, y sin 0, z cos 0): steering angle zero
         }
-        base[j] = (txd + fz) * p.spm + (float)p.delay;
-        base_d[j] = ((double)txd + (double)fz) * (double)p.spm + (double)p.delay;
+        base_d[j] = (txd + (double)fz) * (double)p.spm + (double)p.delay;
+        base[j] = (float)base_d[j];
         // txd = z: depth-only, held in scalar registers (24 VGPRs fewer).  Linear
         // only: the cubic kernel measured slower so (12.3 against 11.8 ms on the
         // 128^3 frame), although it then no longer spills.
