@@ -113,7 +113,7 @@ enum IqFormat { IQ_F32 = 0, IQ_F16 = 1, IQ_BF16 = 2 };
 inline size_t iq_sample_bytes(int fmt) { return fmt == IQ_F32 ? 8 : 4; }
 bool launch_hilbert(const HilbertInput& in, int rows, int S, const float2* d_tw, const float2* d_F, void* out,
                     int out_fmt, float out_scale, hipStream_t st);
-bool hilbert_supported(int S);  // 1 <= S <= 4096
+bool hilbert_supported(int S);  // 1 <= S <= 4096, powers of two up to 65536, others up to 32768
 bool hilbert_pow2(int S);       // powers of two 16..4096 take the radix-16 kernel
 // Tables of the chirp-z (Bluestein) path for other lengths: M = 2^bluestein_log2(S) >= 2S - 1.
 struct BluesteinTables {
@@ -126,6 +126,27 @@ int bluestein_log2(int S);
 bool launch_bluestein_tables(int S, float2* chirp, float2* V, float2* Wt, float2* tw, hipStream_t st);
 bool launch_hilbert_bluestein(const HilbertInput& in, int rows, int S, const BluesteinTables& bt, const float2* d_F,
                               void* out, int ofmt, float oscale, hipStream_t st);
+// Rows of more than 4096 samples: N-point transforms (N = S for powers of
+// two, else the chirp-z length 2^bluestein_log2(S)) as N / 4096 x 4096 steps
+// through HBM, N <= HILBERT_LONG_MAX_N.  Tables: twN = N-point twiddles, tw2 =
+// 4096-point twiddles, and for the chirp-z form chirp (S), V and Wt (N) as
+// BluesteinTables.  The steps run in place in the output when it is complex
+// f32 and S is a power of two; otherwise in a workspace of ws_rows rows of N
+// complex values (hilbert_long_chunk_rows), rows taken in chunks of ws_rows.
+constexpr int HILBERT_LONG_MAX_N = 65536;
+bool hilbert_long(int S);       // S > 4096
+int hilbert_long_n(int S);
+size_t hilbert_long_chunk_rows(int S, int rows);
+bool hilbert_long_in_place(int S, int ofmt);
+struct LongHilbertTables {
+    const float2* twN;
+    const float2* tw2;
+    const float2* chirp;
+    const float2* V;
+    const float2* Wt;
+};
+bool launch_hilbert_long(const HilbertInput& in, int rows, int S, const LongHilbertTables& t, const float2* d_F,
+                         void* out, int ofmt, float oscale, float2* ws, size_t ws_rows, hipStream_t st);
 
 // DAS parameters in kernel form (beamformer.cu:17-98).
 struct DasParams {

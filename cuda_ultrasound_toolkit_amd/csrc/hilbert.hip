@@ -23,6 +23,8 @@
 
 #include <hip/hip_fp16.h>
 
+#include <algorithm>
+
 namespace cut {
 
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
@@ -434,21 +436,24 @@ __global__ void chirp_kernel(float2* chirp, int S)
 
 // out[k] = (1/M) sum_m u[m] exp(-2 pi i m k / M) over the even extension of
 // u[m] = exp(sgn i pi m^2 / S) (m in (-S, S), index m mod M); sgn = +1 gives
-// the spectrum of conj(chirp), -1 that of chirp.
+// the spectrum of conj(chirp), -1 that of chirp.  The terms of m and -m share
+// u[m], so the sum is 1 + 2 sum_{m=1}^{S-1} u[m] cos(2 pi (m k mod M) / M);
+// m^2 mod 2S is carried from m to m + 1 in exact integers.
 __global__ void chirp_spectrum_kernel(float2* out, int S, int M, int sgn)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= M) return;
-    double re = 0.0, im = 0.0;
-    for (int m = -(S - 1); m < S; ++m) {
-        const long long q = ((long long)m * m) % (2LL * S);
-        const long long mm = ((long long)m % M + M) % M;
-        const long long e = (mm * k) % M;
-        // angle / pi = sgn q / S - 2 e / M
+    double re = 1.0, im = 0.0;
+    long long q = 0;  // m^2 mod 2S
+    for (int m = 1; m < S; ++m) {
+        q += 2LL * m - 1;
+        if (q >= 2LL * S) q -= 2LL * S;
+        const unsigned e = ((unsigned)m * (unsigned)k) & (unsigned)(M - 1);
         double sn, cs;
-        sincospi(sgn * (double)q / (double)S - 2.0 * (double)e / (double)M, &sn, &cs);
-        re += cs;
-        im += sn;
+        sincospi(sgn * (double)q / (double)S, &sn, &cs);
+        const double c = 2.0 * cospi(2.0 * (double)e / (double)M);
+        re += c * cs;
+        im += c * sn;
     }
     out[k] = make_float2((float)(re / M), (float)(im / M));
 }
@@ -511,8 +516,10 @@ bool launch_filter_spectrum(const float* d_h, int L, int S, float2* d_F, hipStre
     return true;
 }
 
-bool hilbert_supported(int S) { return S >= 1 && S <= 4096; }
-bool hilbert_pow2(int S) { return S >= 16 && (S & (S - 1)) == 0; }
+bool hilbert_pow2(int S) { return S >= 16 && (S & (S - 1)) == 0 && S <= 4096; }
+bool hilbert_long(int S) { return S > 4096; }
+int hilbert_long_n(int S) { return ((S & (S - 1)) == 0) ? S : 1 << bluestein_log2(S); }
+bool hilbert_supported(int S) { return S >= 1 && (S <= 4096 || hilbert_long_n(S) <= HILBERT_LONG_MAX_N); }
 
 template <int LOGM>
 static bool launch_bluestein_m(const HilbertInput& in, int rows, int S, const BluesteinTables& bt, const float2* F,
@@ -595,6 +602,232 @@ bool launch_hilbert(const HilbertInput& in, int rows, int S, const float2* d_tw,
         case 2048: return launch_hilbert_s<11>(in, rows, d_tw, d_F, out, ofmt, oscale, st);
         case 4096: return launch_hilbert_s<12>(in, rows, d_tw, d_F, out, ofmt, oscale, st);
         default: CUT_FAIL("Hilbert: sample count " << S << " not supported (powers of two 16..4096)");
+    }
+}
+
+
+// ---------------------------------------------------------------------------
+// Rows longer than 4096 samples.  The N-point transforms (N = S for powers of
+// two, else the chirp-z length M = 2^ceil(log2(2S - 1))) are split as
+// N = N1 x 4096, N1 = N / 4096 <= 16, in Bailey's four-step form with the data
+// in HBM between the steps: with n = n2 + 4096 n1 and k = k1 + N1 k2,
+//   X[k1 + N1 k2] = sum_n2 W_4096^(n2 k2) [W_N^(n2 k1) sum_n1 x[n2 + 4096 n1] W_N1^(n1 k1)].
+// long_fwd_pass_kernel does the inner length-N1 DFTs and the twiddles
+// (thread = one n2 of one row, loads coalesced across n2) into B[k1][n2];
+// long_row_kernel transforms each 4096-point segment B[k1][.] in LDS,
+// multiplies the spectrum (index k = k1 + N1 k2) by the spectral weight and
+// filter (or the chirp-z kernel), and transforms back; long_inv_pass_kernel
+// applies the conjugate twiddles and the inverse length-N1 DFTs, which land on
+// natural sample order.  Every step reads and writes the same addresses per
+// thread or workgroup, so the steps run in place in one row-major buffer.
+// Power-of-two rows: forward pass, row kernel (weight, filter), inverse pass
+// into the output.  Other lengths: the chirp-z form of hilbert_bluestein_kernel
+// with each cyclic convolution as these three steps (six launches).
+// ---------------------------------------------------------------------------
+constexpr int LN2_LOG = 12;
+constexpr int LN2 = 1 << LN2_LOG;
+
+enum { LPRE_REAL = 0, LPRE_REAL_CHIRP = 1, LPRE_BUF = 2 };
+enum { LPOST_OUT = 0, LPOST_MID = 1, LPOST_OUT_CHIRP = 2 };
+enum { LROW_HILBERT = 0, LROW_MUL = 1 };
+
+__device__ __forceinline__ void store_fmt(void* out, size_t i, float2 v, int ofmt, float oscale)
+{
+    if (ofmt == IQ_F32) static_cast<float2*>(out)[i] = v;
+    else if (ofmt == IQ_F16) static_cast<__half2*>(out)[i] = __floats2half2_rn(v.x * oscale, v.y * oscale);
+    else static_cast<uint32_t*>(out)[i] = bf16_bits(v.x) | (bf16_bits(v.y) << 16);
+}
+
+// grid (LN2 / 256, rows of the chunk); rows row0 + blockIdx.y.
+template <int N1, int KIND, int PRE>
+__global__ __launch_bounds__(256) void long_fwd_pass_kernel(HilbertDev in, int row0, int S,
+                                                            const float2* __restrict__ twN,
+                                                            const float2* __restrict__ chirp, float2* buf)
+{
+    constexpr int N = N1 * LN2;
+    const int n2 = blockIdx.x * 256 + threadIdx.x;
+    float2* brow = buf + (size_t)blockIdx.y * N;
+    float2 u[N1];
+#pragma unroll
+    for (int n1 = 0; n1 < N1; ++n1) {
+        const int n = n2 + LN2 * n1;
+        if constexpr (PRE == LPRE_BUF) {
+            u[n1] = brow[n];
+        } else {
+            const float x = n < S ? load_in<KIND>(in, row0 + (int)blockIdx.y, S, n) : 0.0f;
+            if constexpr (PRE == LPRE_REAL_CHIRP) {
+                const float2 c = n < S ? chirp[n] : make_float2(0.0f, 0.0f);
+                u[n1] = make_float2(x * c.x, x * c.y);
+            } else {
+                u[n1] = make_float2(x, 0.0f);
+            }
+        }
+    }
+    dft<N1, false>(u);
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) {
+        float2 v = u[k1];
+        if (k1 > 0) v = cmul(v, twN[n2 * k1]);  // n2 k1 < N
+        brow[k1 * LN2 + n2] = v;
+    }
+}
+
+// grid (N1, rows of the chunk), 256 threads: segment k1 = blockIdx.x of a row.
+template <int MODE, bool FILTER>
+__global__ __launch_bounds__(256) void long_row_kernel(float2* buf, int N1, int S, const float2* __restrict__ tw,
+                                                       const float2* __restrict__ G, const float2* __restrict__ F)
+{
+    __shared__ float2 lds[LN2 + LN2 / 16];
+    const int N = N1 * LN2;
+    const int k1 = blockIdx.x;
+    float2* seg = buf + (size_t)blockIdx.y * N + (size_t)k1 * LN2;
+    const int tid = threadIdx.x;
+    auto load_g = [&](int i) { return seg[i]; };
+    const float w1 = 1.5f / (float)S, w0 = 0.75f / (float)S;
+    const int half = S / 2;
+    auto store_mul = [&](int k2, float2 v) {
+        const int k = k1 + N1 * k2;
+        if constexpr (MODE == LROW_HILBERT) {
+            if (k > half) {
+                v = make_float2(0.0f, 0.0f);
+            } else {
+                const float w = k == 0 ? w0 : w1;
+                v = make_float2(v.x * w, v.y * w);
+                if constexpr (FILTER) v = cmul(v, F[k]);
+            }
+        } else {
+            v = cmul(v, G[k]);
+        }
+        lds[lds_idx(k2)] = v;
+    };
+    fft_passes<LN2_LOG, false>(tid, tw, lds, load_g, false, store_mul, false);
+    auto load_lds = [&](int i) { return lds[lds_idx(i)]; };
+    auto store_g = [&](int i, float2 v) { seg[i] = v; };
+    const float2* tw2 = tw;
+    __asm__ volatile("" : "+s"(tw2));
+    fft_passes<LN2_LOG, true>(tid, tw2, lds, load_lds, true, store_g, true);
+}
+
+// grid (LN2 / 256, rows of the chunk).  POST: LPOST_OUT writes the row
+// (samples < S) to out in format ofmt; LPOST_MID forms the chirp-z middle
+// step in place (X = chirp c, weight and filter for k <= S/2, times
+// conj(chirp), zero from S/2 + 1 on); LPOST_OUT_CHIRP writes conj(chirp) y.
+template <int N1, int POST, bool FILTER>
+__global__ __launch_bounds__(256) void long_inv_pass_kernel(int row0, int S, const float2* __restrict__ twN,
+                                                            const float2* __restrict__ chirp,
+                                                            const float2* __restrict__ F, float2* buf, void* out,
+                                                            int ofmt, float oscale)
+{
+    constexpr int N = N1 * LN2;
+    const int n2 = blockIdx.x * 256 + threadIdx.x;
+    float2* brow = buf + (size_t)blockIdx.y * N;
+    float2 u[N1];
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) {
+        const float2 v = brow[k1 * LN2 + n2];
+        u[k1] = k1 > 0 ? cmul_tw<true>(v, twN[n2 * k1]) : v;
+    }
+    dft<N1, true>(u);
+    const size_t orow = (size_t)(row0 + (int)blockIdx.y) * S;
+    const float w1 = 1.5f / (float)S, w0 = 0.75f / (float)S;
+    const int half = S / 2;
+#pragma unroll
+    for (int n1 = 0; n1 < N1; ++n1) {
+        const int n = n2 + LN2 * n1;
+        float2 v = u[n1];
+        if constexpr (POST == LPOST_OUT) {
+            if (n < S) store_fmt(out, orow + n, v, ofmt, oscale);
+        } else if constexpr (POST == LPOST_OUT_CHIRP) {
+            if (n < S) {
+                const float2 c = chirp[n];
+                store_fmt(out, orow + n, cmul(v, make_float2(c.x, -c.y)), ofmt, oscale);
+            }
+        } else {
+            float2 r = make_float2(0.0f, 0.0f);
+            if (n <= half) {
+                const float2 c = chirp[n];
+                float2 X = cmul(v, c);
+                const float w = n == 0 ? w0 : w1;
+                X = make_float2(X.x * w, X.y * w);
+                if constexpr (FILTER) X = cmul(X, F[n]);
+                r = cmul(X, make_float2(c.x, -c.y));
+            }
+            brow[n] = r;
+        }
+    }
+}
+
+size_t hilbert_long_chunk_rows(int S, int rows)
+{
+    const size_t per_row = (size_t)hilbert_long_n(S) * sizeof(float2);
+    const size_t cap = (size_t)512 << 20;  // workspace bound
+    return std::max<size_t>(1, std::min<size_t>((size_t)rows, cap / per_row));
+}
+
+bool hilbert_long_in_place(int S, int ofmt) { return ofmt == IQ_F32 && (S & (S - 1)) == 0; }
+
+template <int N1>
+static bool launch_long_n1(const HilbertInput& in, int rows, int S, const LongHilbertTables& t, const float2* F,
+                           void* out, int ofmt, float oscale, float2* ws, size_t ws_rows, hipStream_t st)
+{
+    HilbertDev d{in.ptr, in.stride, in.map, in.R0, in.C};
+    const bool pow2 = (S & (S - 1)) == 0;
+    const bool inplace = hilbert_long_in_place(S, ofmt);
+    const int chunk = inplace ? rows : (int)ws_rows;
+    if (!inplace && (!ws || chunk <= 0)) CUT_FAIL("Hilbert: no workspace for rows of " << S << " samples");
+    for (int r0 = 0; r0 < rows; r0 += chunk) {
+        const int nr = std::min(chunk, rows - r0);
+        float2* buf = inplace ? static_cast<float2*>(out) + (size_t)r0 * S : ws;
+        const dim3 gp(LN2 / 256, nr), gr(N1, nr);
+        auto fwd = [&](int pre) {
+            if (pre == LPRE_BUF) {
+                long_fwd_pass_kernel<N1, 0, LPRE_BUF><<<gp, 256, 0, st>>>(d, r0, S, t.twN, t.chirp, buf);
+                return;
+            }
+#define CUT_LF(K, P) long_fwd_pass_kernel<N1, K, P><<<gp, 256, 0, st>>>(d, r0, S, t.twN, t.chirp, buf)
+            if (pre == LPRE_REAL) {
+                if (in.kind == 0) CUT_LF(0, LPRE_REAL); else if (in.kind == 1) CUT_LF(1, LPRE_REAL); else CUT_LF(2, LPRE_REAL);
+            } else {
+                if (in.kind == 0) CUT_LF(0, LPRE_REAL_CHIRP); else if (in.kind == 1) CUT_LF(1, LPRE_REAL_CHIRP);
+                else CUT_LF(2, LPRE_REAL_CHIRP);
+            }
+#undef CUT_LF
+        };
+        if (pow2) {
+            fwd(LPRE_REAL);
+            if (F) long_row_kernel<LROW_HILBERT, true><<<gr, 256, 0, st>>>(buf, N1, S, t.tw2, nullptr, F);
+            else long_row_kernel<LROW_HILBERT, false><<<gr, 256, 0, st>>>(buf, N1, S, t.tw2, nullptr, nullptr);
+            long_inv_pass_kernel<N1, LPOST_OUT, false><<<gp, 256, 0, st>>>(r0, S, t.twN, t.chirp, nullptr, buf, out,
+                                                                          ofmt, oscale);
+        } else {
+            fwd(LPRE_REAL_CHIRP);
+            long_row_kernel<LROW_MUL, false><<<gr, 256, 0, st>>>(buf, N1, S, t.tw2, t.V, nullptr);
+            if (F) long_inv_pass_kernel<N1, LPOST_MID, true><<<gp, 256, 0, st>>>(r0, S, t.twN, t.chirp, F, buf, out,
+                                                                                ofmt, oscale);
+            else long_inv_pass_kernel<N1, LPOST_MID, false><<<gp, 256, 0, st>>>(r0, S, t.twN, t.chirp, nullptr, buf,
+                                                                               out, ofmt, oscale);
+            fwd(LPRE_BUF);
+            long_row_kernel<LROW_MUL, false><<<gr, 256, 0, st>>>(buf, N1, S, t.tw2, t.Wt, nullptr);
+            long_inv_pass_kernel<N1, LPOST_OUT_CHIRP, false><<<gp, 256, 0, st>>>(r0, S, t.twN, t.chirp, nullptr, buf,
+                                                                                out, ofmt, oscale);
+        }
+        CUT_HIP_CHECK(hipGetLastError());
+    }
+    return true;
+}
+
+bool launch_hilbert_long(const HilbertInput& in, int rows, int S, const LongHilbertTables& t, const float2* d_F,
+                         void* out, int ofmt, float oscale, float2* ws, size_t ws_rows, hipStream_t st)
+{
+    if (rows <= 0) return true;
+    if (ofmt != IQ_F32 && ofmt != IQ_F16 && ofmt != IQ_BF16) CUT_FAIL("Hilbert: unknown output format " << ofmt);
+    if (!hilbert_long(S) || !hilbert_supported(S)) CUT_FAIL("Hilbert: sample count " << S << " not supported");
+    switch (hilbert_long_n(S) / LN2) {
+        case 2: return launch_long_n1<2>(in, rows, S, t, d_F, out, ofmt, oscale, ws, ws_rows, st);
+        case 4: return launch_long_n1<4>(in, rows, S, t, d_F, out, ofmt, oscale, ws, ws_rows, st);
+        case 8: return launch_long_n1<8>(in, rows, S, t, d_F, out, ofmt, oscale, ws, ws_rows, st);
+        case 16: return launch_long_n1<16>(in, rows, S, t, d_F, out, ofmt, oscale, ws, ws_rows, st);
+        default: CUT_FAIL("Hilbert: sample count " << S << " not supported");
     }
 }
 

@@ -155,9 +155,14 @@ bool RfPipeline::init(uint raw_len, uint raw_channels, uint S, uint C, uint T, i
         scale_ = 1.0f / (float)T;
     }
     // FFT plan: twiddles for power-of-two lengths, chirp-z tables for the
-    // others (lengths beyond 4096 fail at hilbert())
-    chirp_.release(); bV_.release(); bW_.release();
-    if (hilbert_pow2((int)S)) {
+    // others; rows longer than 4096 also take the 4096-point twiddles of their
+    // row steps (unsupported lengths fail at hilbert())
+    chirp_.release(); bV_.release(); bW_.release(); tw2_.release(); long_ws_.release();
+    if (hilbert_long((int)S) && hilbert_supported((int)S)) {
+        if (!tw2_.ensure((size_t)4096 * sizeof(float2)) || !launch_twiddles(tw2_.as<float2>(), 4096, stream()))
+            return false;
+    }
+    if (hilbert_pow2((int)S) || (hilbert_long((int)S) && hilbert_supported((int)S) && (S & (S - 1)) == 0)) {
         if (!tw_.ensure((size_t)S * sizeof(float2))) return false;
         if (!launch_twiddles(tw_.as<float2>(), (int)S, stream())) return false;
     } else if (hilbert_supported((int)S)) {
@@ -191,7 +196,7 @@ bool RfPipeline::deinit()
     }
     init_ = false;
     h8_.release(); hf_.release(); rowsum_.release(); tw_.release();
-    chirp_.release(); bV_.release(); bW_.release();
+    chirp_.release(); bV_.release(); bW_.release(); tw2_.release(); long_ws_.release();
     filter_spec_.release(); filter_taps_.release();
     filter_host_.clear();
     filter_on_ = false;
@@ -276,6 +281,18 @@ bool RfPipeline::hilbert(const HilbertInput& in, void* d_out, int fmt, float sca
     if (!hilbert_supported((int)S_)) CUT_FAIL("Hilbert: sample count " << S_ << " is not supported by the FFT.");
     const float2* F = filter_on_ ? filter_spec_.as<float2>() : nullptr;
     if (hilbert_pow2((int)S_)) return launch_hilbert(in, (int)(C_ * T_), (int)S_, tw_.as<float2>(), F, d_out, fmt, scale, st);
+    if (hilbert_long((int)S_)) {
+        const int rows = (int)(C_ * T_);
+        size_t ws_rows = 0;
+        if (!hilbert_long_in_place((int)S_, fmt)) {
+            ws_rows = hilbert_long_chunk_rows((int)S_, rows);
+            if (!long_ws_.ensure(std::max(long_ws_.bytes, ws_rows * hilbert_long_n((int)S_) * sizeof(float2))))
+                return false;
+        }
+        const LongHilbertTables t{tw_.as<float2>(), tw2_.as<float2>(), chirp_.as<float2>(), bV_.as<float2>(),
+                                  bW_.as<float2>()};
+        return launch_hilbert_long(in, rows, (int)S_, t, F, d_out, fmt, scale, long_ws_.as<float2>(), ws_rows, st);
+    }
     const BluesteinTables bt{chirp_.as<float2>(), bV_.as<float2>(), bW_.as<float2>(), tw_.as<float2>()};
     return launch_hilbert_bluestein(in, (int)(C_ * T_), (int)S_, bt, F, d_out, fmt, scale, st);
 }
@@ -293,8 +310,8 @@ float RfPipeline::fp16_scale() const
 {
     // |decoded| <= 32768; filtering multiplies the bound by sum |h|; the
     // analytic signal of a length-S row (w_k = 0.75/S, 1.5/S) stays below
-    // 0.75 (1 + (2/pi) ln S + 1) times the filtered bound, under 8 for
-    // S <= 4096.  scale = 2^-k with 32768 * 8 * max(1, sum |h|) * scale <= 32768.
+    // 0.75 (1 + (2/pi) ln S + 1) times the filtered bound, under 8 for every
+    // supported S (6.8 at 65536).  scale = 2^-k with 32768 * 8 * max(1, sum |h|) * scale <= 32768.
     double l1 = 0.0;
     if (filter_on_)
         for (float v : filter_host_) l1 += std::fabs((double)v);

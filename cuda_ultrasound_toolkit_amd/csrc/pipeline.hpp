@@ -65,6 +65,7 @@ private:
     float scale_ = 1.0f;
     DeviceBuffer tw_;
     DeviceBuffer chirp_, bV_, bW_;  // chirp-z tables (lengths that are not powers of two)
+    DeviceBuffer tw2_, long_ws_;    // rows > 4096: 4096-point twiddles, workspace of the HBM-staged steps
     DeviceBuffer filter_taps_, filter_spec_;
     std::vector<float> filter_host_;
     bool filter_on_ = false;

@@ -71,8 +71,11 @@ REJECTIONS = {
     "zero_dims": (lambda b: b.dec_data_dim.__setitem__(0, 0), "Invalid dimensions for RF raw data"),
     "raw_short": (lambda b: b.rf_raw_dim.__setitem__(1, 64), "rf_raw_dim needs"),
     "out_short": (lambda b: None, "Output span holds 80 bytes"),
-    "fft_len": (lambda b: (b.dec_data_dim.__setitem__(0, 5000), b.rf_raw_dim.__setitem__(0, 32 * 5000)),
-                "Sample count 5000 is not supported"),
+    # rows up to 65536 samples (powers of two) or 32768 (other lengths) are supported
+    "fft_len": (lambda b: (b.dec_data_dim.__setitem__(0, 40000), b.rf_raw_dim.__setitem__(0, 32 * 40000)),
+                "Sample count 40000 is not supported"),
+    "fft_len_pow2": (lambda b: (b.dec_data_dim.__setitem__(0, 131072), b.rf_raw_dim.__setitem__(0, 32 * 131072)),
+                     "Sample count 131072 is not supported"),
 }
 
 

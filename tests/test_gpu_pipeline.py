@@ -446,3 +446,84 @@ def test_beamform_f32_matlab_shim(lib, orc):
         err = rel_l2(out, orc.pipeline(acq))
         print(f"  rel L2 {err:.3e}")
         assert err < 1e-4
+
+
+@pytest.mark.parametrize("S,taps,rows", [(4097, 0, 12), (5000, 64, 12), (6000, 0, 7), (8192, 64, 12), (8192, 0, 5),
+                                         (12289, 17, 6), (16384, 64, 6), (32768, 0, 3), (65536, 64, 3),
+                                         (32767, 3, 2),
+                                         # 1030 rows of the 65536-point chirp-z length: two workspace chunks
+                                         (20000, 5, 1030)])
+def test_hilbert_long_rows(torch_dev, orc, S, taps, rows):
+    """Rows longer than 4096 samples (the reference plans cuFFT for any length,
+    hilbert_handler.cu:30-53): N = S (powers of two) or the chirp-z length as
+    N/4096 x 4096 steps through HBM, against the FP64 oracle."""
+    import torch
+    print(f"hilbert long S={S} taps={taps} rows={rows}", flush=True)
+    assert api.rf_init((rows * S, 1), (S, 1, rows), L.NONE)
+    rng = np.random.default_rng(S + taps)
+    x = rng.standard_normal((rows, S)).astype(np.float32) * 1000
+    h = rng.standard_normal(taps).astype(np.float32) if taps else None
+    assert api.rf_set_match_filter(h)
+    out = torch.full((rows * S * 2,), np.nan, dtype=torch.float32, device=torch_dev)
+    assert api.rf_hilbert(_t(x, torch_dev), 1, out)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy().view(np.complex64).reshape(rows, S)
+    want = orc.hilbert(x.astype(np.float64), None if h is None else h.astype(np.float64))
+    err = rel_l2(got, want)
+    worst = max(rel_l2(got[r], want[r]) for r in range(min(rows, 16)))
+    print(f"  rel L2 {err:.2e}, worst row {worst:.2e}")
+    assert np.all(np.isfinite(got))
+    assert err < 1e-5 and worst < 1e-5
+    if rows <= 12:  # strided input: real parts of a complex buffer
+        xc = np.zeros((rows, S), np.complex64)
+        xc.real = x
+        xc.imag = -5.0
+        out2 = torch.zeros_like(out)
+        assert api.rf_hilbert(_t(xc.view(np.float32), torch_dev), 2, out2)
+        torch.cuda.synchronize()
+        assert np.array_equal(out2.cpu().numpy(), out.cpu().numpy())
+    assert api.rf_set_match_filter(None)
+
+
+@pytest.mark.parametrize("S", [8192, 6000])
+@pytest.mark.parametrize("fmt,tol", [(1, 3e-4), (2, 2.5e-3)])
+def test_hilbert_long_rows_staged_formats(torch_dev, orc, S, fmt, tol):
+    """FP16 (x 2^-k) and BF16 IQ from long rows (the workspace path for powers
+    of two too), against the oracle at the staged formats' bars."""
+    import torch
+    print(f"hilbert long staged S={S} fmt={fmt}", flush=True)
+    rows = 6
+    assert api.rf_init((rows * S, 1), (S, 1, rows), L.NONE)
+    rng = np.random.default_rng(S + fmt)
+    x = np.round(rng.standard_normal((rows, S)) * 3000).astype(np.float32)
+    h = configs.chirp_filter(64)
+    assert api.rf_set_match_filter(h)
+    out = torch.zeros(rows * S, dtype=torch.int32, device=torch_dev)
+    ok, scale = api.rf_hilbert_format(_t(x, torch_dev), 1, out, fmt)
+    assert ok
+    torch.cuda.synchronize()
+    assert api.rf_set_match_filter(None)
+    raw = out.cpu().numpy().view(np.uint32)
+    if fmt == 1:
+        got = raw.view(np.float16).astype(np.float32).reshape(rows, S, 2) / scale
+    else:
+        got = np.stack([(raw << 16).view(np.float32), (raw & 0xFFFF0000).view(np.float32)], -1).reshape(rows, S, 2)
+    got = got[..., 0] + 1j * got[..., 1]
+    want = orc.hilbert(x.astype(np.float64), h.astype(np.float64))
+    err = rel_l2(got, want)
+    print(f"  rel L2 {err:.2e}")
+    assert err < tol
+
+
+@pytest.mark.parametrize("S,encoding", [(6000, L.HADAMARD), (8192, L.HADAMARD), (8192, L.NONE), (5001, L.NONE)])
+def test_beamform_long_rows(lib, orc, S, encoding):
+    """The whole path with rows longer than 4096 samples (decode = NONE takes
+    the convert-fused first step) against the oracle."""
+    print(f"beamform rca_small S={S} encoding={encoding}", flush=True)
+    acq = configs.rca_small(seed=12, S=S, encoding=encoding)
+    acq.bp.decode = encoding
+    got = api.beamform_volume(acq.raw, acq.bp)
+    assert got is not None
+    err = rel_l2(got, orc.pipeline(acq))
+    print(f"  rel L2 {err:.2e}")
+    assert err < 1e-4
