@@ -56,10 +56,9 @@ bool validate_motion(const NccMotionParameters& p, size_t image_bytes, size_t ma
     if (map_bytes < (size_t)d.gx * d.gy * d.frames * sizeof(float4))
         CUT_FAIL("motion_detection: motion map span holds " << map_bytes << " bytes, "
                  << (size_t)d.gx * d.gy * d.frames * 16 << " needed");
+    if (d.mx < 0 || d.my < 0) CUT_FAIL("motion_detection: negative search margins");
     const long long vw = std::min(2LL * d.mx + 1, (long long)d.W), vh = std::min(2LL * d.my + 1, (long long)d.H);
-    if (vw * vh > NCC2D_MAP_MAX)
-        CUT_FAIL("motion_detection: a search of (2*" << d.mx << "+1) x (2*" << d.my << "+1) offsets exceeds "
-                 << NCC2D_MAP_MAX);
+    if (vw * vh > 0x7fffffffLL) CUT_FAIL("motion_detection: a search of " << vw << " x " << vh << " offsets is too large");
     return true;
 }
 

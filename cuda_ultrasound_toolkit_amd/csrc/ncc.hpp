@@ -5,7 +5,8 @@
 
 namespace cut {
 
-// Largest ZNCC map (valid offsets per grid point) one workgroup holds in LDS.
+// Longest ZNCC map prefix (valid offsets per grid point) a workgroup holds in
+// LDS; entries beyond it that the peak search reads are evaluated on demand.
 constexpr int NCC2D_MAP_MAX = 8192;
 
 // NccMotionParameters in kernel form (cuda_beamformer_parameters.h:91-107).

@@ -4,8 +4,8 @@
 //   with block_match_kernels.cu:7-240 (NPP valid-region ZNCC, CUB top-32 sort,
 //   3x3 paraboloid peak test).  The reference launches three kernels per grid
 //   point from a host loop; here one workgroup owns one (frame, grid point):
-//   it computes the whole ZNCC map into LDS (FP64 window sums and energy
-//   normalisation), selects the 32 largest of the first 2048 map entries in the
+//   it computes the ZNCC map entries the search reads into LDS (FP64 window
+//   sums and energy normalisation), selects the 32 largest of the first 2048 map entries in the
 //   sort's order (value descending, index ascending on ties, missing entries
 //   -1), and runs the peak test on lanes 0..31 of the first wave with the same
 //   32-lane shuffle-down max tree, in FP32 like the reference.
@@ -101,8 +101,9 @@ __global__ __launch_bounds__(BM_THREADS) void block_match2d_kernel(const float* 
     }
     te = block_sum_d(te, red);
 
-    // ZNCC map
-    for (int s = tid; s < total; s += BM_THREADS) {
+    // ZNCC map entry s (offset (s % vw, s / vw)), the same arithmetic wherever
+    // it is evaluated
+    auto entry = [&](int s) -> float {
         const int u = s % vw, v = s / vw;
         double ss = 0.0, s2 = 0.0, ct = 0.0;
         for (int y = 0; y < r.th; ++y) {
@@ -117,10 +118,17 @@ __global__ __launch_bounds__(BM_THREADS) void block_match2d_kernel(const float* 
         }
         const double se = s2 - ss * ss / n;
         const double den = sqrt(te * (se > 0.0 ? se : 0.0));
-        map[s] = den > 0.0 ? (float)(ct / den) : 0.0f;
-    }
+        return den > 0.0 ? (float)(ct / den) : 0.0f;
+    };
+    // The peak search reads the first 2048 entries, the 3 x 3 neighbourhoods
+    // of its candidates (at most one map row further) and the no-shift entry:
+    // the map prefix [0, P) is held in LDS, any other entry that is read is
+    // evaluated where it is needed, so maps of any size are searched.
+    const int P = min(total, min(NCC2D_MAP_MAX, BM_SORT + vw + 1));
+    for (int s = tid; s < P; s += BM_THREADS) map[s] = entry(s);
     for (int i = tid; i < BM_SORT; i += BM_THREADS) taken[i] = 0;
     __syncthreads();
+    auto map_at = [&](int s) { return s < P ? map[s] : entry(s); };
 
     // top 32 of the first 2048 entries (missing ones -1), value desc, index asc
     for (int c = 0; c < BM_CAND; ++c) {
@@ -158,7 +166,7 @@ __global__ __launch_bounds__(BM_THREADS) void block_match2d_kernel(const float* 
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
         const int yy = py + k / 3 - 1, xx = px + k % 3 - 1;
-        vals[k] = (yy < 0 || yy >= vh || xx < 0 || xx >= vw) ? 0.0f : map[yy * vw + xx];
+        vals[k] = (yy < 0 || yy >= vh || xx < 0 || xx >= vw) ? 0.0f : map_at(yy * vw + xx);
     }
     float co[6];
 #pragma unroll
@@ -181,7 +189,7 @@ __global__ __launch_bounds__(BM_THREADS) void block_match2d_kernel(const float* 
     sx = fminf(fmaxf(sx, -1.0f), 1.0f) * use;
     sy = fminf(fmaxf(sy, -1.0f), 1.0f) * use;
     float ox = sx + (float)px - (float)nsx, oy = sy + (float)py - (float)nsy;
-    const float no_shift_peak = map[nsy * vw + nsx];
+    const float no_shift_peak = map_at(nsy * vw + nsx);
     const float thr = no_shift_peak * (1.0f + p.rel_thr);
     float peak = cand_v[tid];
     if (min_sharp < p.min_var || s0 >= 0.0f || peak > 1.0f || peak < thr) peak = -2.0f;

@@ -108,12 +108,11 @@ def test_motion_detection_rejects_invalid_parameters(lib, capfd):
     assert "image span holds 8000 bytes" in capfd.readouterr().err
     assert api.motion_detection(imgs, maps[:1], ok) is False           # map span too small
     assert "motion map span holds 320 bytes" in capfd.readouterr().err
-    # more offsets than one workgroup's map holds (the search is clamped to
-    # the image, so the image must be larger than the window for this)
-    big = np.zeros((2, 200, 200), np.float32)
-    p = api.motion_params(200, 200, 2, (5, 4), 8, 8, (100, 100))
-    assert api.motion_detection(big, np.zeros((2, 4, 5, 4), np.float32), p) is False
-    assert "(2*100+1) x (2*100+1) offsets exceeds" in capfd.readouterr().err
+    # margins whose sign bit is set (uint fields) are refused, any search size is not
+    p = api.motion_params(50, 40, 3, (5, 4), 8, 8, (4, 4))
+    p.search_margins[0] = 0x80000000
+    assert api.motion_detection(imgs, maps, p) is False
+    assert "negative search margins" in capfd.readouterr().err
 
 
 def test_corr_images_rejects_invalid_dims(lib, capfd):

@@ -33,7 +33,8 @@ def compare(got, want):
     assert np.allclose(got[..., :2], want[..., :2], atol=2e-3), np.abs(got[..., :2] - want[..., :2]).max()
 
 
-@pytest.mark.parametrize("case", ["basic", "edges", "nosub", "neighbour", "rect", "noisy", "wide"])
+@pytest.mark.parametrize("case", ["basic", "edges", "nosub", "neighbour", "rect", "noisy", "wide", "huge",
+                                  "very_wide"])
 def test_motion_detection_matches_oracle(lib, orc, case):
     print(f"motion {case}", flush=True)
     if case == "basic":
@@ -57,6 +58,16 @@ def test_motion_detection_matches_oracle(lib, orc, case):
         # (dy = +19) lies in the truncated rows
         imgs = frames_with_shifts((128, 136), [(0, 0), (3, -2), (5, 19)], 9)
         p = api.motion_params(136, 128, 3, (4, 4), 24, 16, (24, 24), abs_thr=0.3, rel_thr=0.0)
+    elif case == "huge":
+        # 97^2 = 9409 offsets: more than the LDS map prefix; the no-shift entry
+        # (index 4656) lies beyond the entries the search keeps in LDS
+        imgs = frames_with_shifts((176, 180), [(0, 0), (4, 3), (-6, 30)], 10)
+        p = api.motion_params(180, 176, 3, (3, 3), 40, 16, (48, 48), abs_thr=0.3, rel_thr=0.0)
+    elif case == "very_wide":
+        # 7001 x 3 offsets: the first 2048 entries are in map row 0, their lower
+        # neighbours (row 1) beyond the LDS prefix of 8192 entries
+        imgs = frames_with_shifts((40, 7100), [(0, 0), (-2, 0)], 11)
+        p = api.motion_params(7100, 40, 2, (3, 2), 1800, 16, (3500, 1), abs_thr=0.3, rel_thr=0.0)
     else:
         imgs = frames_with_shifts((72, 72), [(0, 0), (2, -3)], 6, noise=0.2)
         p = api.motion_params(72, 72, 2, (5, 5), 14, 16, (5, 5), abs_thr=0.2, rel_thr=0.01, min_var=0.01)
