@@ -11,6 +11,10 @@ from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "libcuda_toolkit.so"
+# CUT_LIBRARY selects another build of the same library (the host-sanitizer
+# build that the CPU suite runs its host-side tests against).
+if os.environ.get("CUT_LIBRARY"):
+    LIB_PATH = Path(os.environ["CUT_LIBRARY"]).resolve()
 
 MAX_ACQ_COUNT = 256
 MAX_CHANNEL_COUNT = 256

@@ -92,5 +92,65 @@ def build_transfer() -> None:
     os.replace(tmp, TRANSFER_CLIENT)
 
 
+# ---------------------------------------------------------------------------
+# Host-sanitizer build: the host-side sources (.cpp: API layers, validation,
+# RF state, DAS constants, batch set-up, synthetic generator, transfer server
+# and client) instrumented with AddressSanitizer and UndefinedBehaviorSanitizer
+# (host code only: each -fsanitize= follows -Xarch_host, so no GPU code is
+# instrumented), linked with the regular kernel objects into SAN_DIR.  The CPU
+# suite runs its host-side tests against it (tests/test_host_sanitizers.py).
+# ---------------------------------------------------------------------------
+SAN_DIR = HERE / "_build_san"
+SAN_HOST = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+            "-Xarch_host", "-fno-omit-frame-pointer", "-Xarch_host", "-fno-sanitize-recover=undefined"]
+CLANG = "/opt/rocm/llvm/bin/clang"
+
+
+def sanitizer_runtime() -> Path | None:
+    """The clang AddressSanitizer runtime the sanitized build links against."""
+    hits = sorted(Path("/opt/rocm/llvm/lib/clang").glob("*/lib/linux/libclang_rt.asan-x86_64.so"))
+    return hits[-1] if hits else None
+
+
+def build_sanitized(verbose: bool = True) -> Path:
+    rt = sanitizer_runtime()
+    if rt is None:
+        raise RuntimeError("no clang AddressSanitizer runtime under /opt/rocm/llvm")
+    SAN_DIR.mkdir(exist_ok=True)
+    hosts = sorted(CSRC.glob("*.cpp"))
+
+    def comp(src: Path) -> Path:
+        obj = SAN_DIR / (src.name + ".o")
+        cmd = [HIPCC, "-O1", "-g", "-fPIC", "-std=c++20", "-fvisibility=hidden", f"--offload-arch={ARCH}",
+               "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-x", "hip", *SAN_HOST,
+               "-c", str(src), "-o", str(obj)]
+        _run(cmd, f"sanitizer build of {src.name}")
+        return obj
+
+    with cf.ThreadPoolExecutor(min(len(hosts), 8)) as ex:
+        objs = list(ex.map(comp, hosts))
+    kernels = [OBJ / (p.name + ".o") for p in sorted(CSRC.glob("*.hip"))]
+    for k in kernels:
+        if not k.exists():
+            raise RuntimeError(f"{k} missing: build() first")
+    lib = SAN_DIR / "libcuda_toolkit.so"
+    rpath = f"-Wl,-rpath,{rt.parent}"
+    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", *SAN_HOST, "-shared-libsan", rpath, "-o", str(lib),
+          *map(str, objs), *map(str, kernels), "-lpthread"], "sanitizer link")
+    tdir = CSRC / "transfer"
+    _run([HIPCC, "-O1", "-g", "-std=c++20", "-Wall", f"-I{INCLUDE}", f"-I{tdir}", *SAN_HOST, "-shared-libsan", rpath,
+          str(tdir / "transfer_server.cpp"), "-o", str(SAN_DIR / "cut_transfer_server"), f"-L{SAN_DIR}",
+          "-lcuda_toolkit", "-Wl,-rpath,$ORIGIN"], "sanitizer build of the transfer server")
+    _run([CLANG, "-O1", "-g", "-std=c11", "-Wall", "-fPIC", "-shared", "-fvisibility=hidden", f"-I{INCLUDE}",
+          f"-I{tdir}", "-fsanitize=address,undefined", "-fno-sanitize-recover=undefined", "-shared-libsan", rpath,
+          str(tdir / "transfer_client.c"), "-o", str(SAN_DIR / "libcut_transfer.so"), "-lrt"],
+         "sanitizer build of the transfer client")
+    if verbose:
+        print(f"built host-sanitizer variants in {SAN_DIR}")
+    return lib
+
+
 if __name__ == "__main__":
     build(incremental="--incremental" in sys.argv)
+    if "--sanitize" in sys.argv:
+        build_sanitized()

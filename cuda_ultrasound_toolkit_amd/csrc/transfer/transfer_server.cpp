@@ -149,7 +149,13 @@ private:
         }
         const auto* in = static_cast<const uint8_t*>(data_.ptr);
         const size_t in_size = (size_t)m.data_size;
-        switch (m.opcode) {
+        // the opcode field is read as the int it is on the wire: a client may
+        // send a value that is no CudaCommand, and loading that through the
+        // enum type is undefined
+        int opcode;
+        static_assert(sizeof(opcode) == sizeof(m.opcode));
+        std::memcpy(&opcode, &m.opcode, sizeof(opcode));
+        switch (opcode) {
         case BEAMFORM_VOLUME: {  // test_app.cpp:101-154
             const CudaBeamformerParameters bp = params_.beamformerParameters;
             respond(ACK, 0);
@@ -208,7 +214,7 @@ private:
             return;
         }
         default:  // SVD_FILTER (not implemented by the reference either), unknown opcodes
-            std::cerr << "cut_transfer_server: unsupported command " << (int)m.opcode << "\n";
+            std::cerr << "cut_transfer_server: unsupported command " << opcode << "\n";
             respond(ERR, 0);
             return;
         }

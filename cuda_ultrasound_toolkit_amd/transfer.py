@@ -19,8 +19,10 @@ import numpy as np
 from . import _lib as L
 
 HERE = Path(__file__).resolve().parent
-SERVER = HERE / "cut_transfer_server"
-CLIENT = HERE / "libcut_transfer.so"
+# CUT_TRANSFER_SERVER / CUT_TRANSFER_CLIENT select other builds of the two
+# (the host-sanitizer build)
+SERVER = Path(os.environ.get("CUT_TRANSFER_SERVER", HERE / "cut_transfer_server")).resolve()
+CLIENT = Path(os.environ.get("CUT_TRANSFER_CLIENT", HERE / "libcut_transfer.so")).resolve()
 
 # CudaCommand (communication_params.h:21-30)
 ERR, ACK, SUCCESS, BEAMFORM_VOLUME, SVD_FILTER, NCC_MOTION_DETECT, CORR_IMAGES = range(7)
