@@ -183,6 +183,9 @@ def load() -> C.CDLL:
     _proto(lib, "cut_beamform_device", B, [V, C.c_size_t, V, C.c_size_t, P(CudaBeamformerParameters)])
     _proto(lib, "cut_beamform_batch", B, [P(CutFrame), C.c_int, C.c_int, P(C.c_double)])
     _proto(lib, "cut_set_autotune", B, [C.c_int])
+    _proto(lib, "cut_set_fft_tile", B, [C.c_int])
+    _proto(lib, "cut_tuning_save", B, [C.c_char_p])
+    _proto(lib, "cut_tuning_load", B, [C.c_char_p])
     _proto(lib, "cut_batch_launch_shape", B, [P(C.c_int), P(C.c_int)])
     _proto(lib, "cut_rf_init", B, [P(u), P(u), C.c_int])
     _proto(lib, "cut_rf_set_channel_mapping", B, [P(C.c_int16), u])
@@ -227,6 +230,7 @@ def exported_symbols() -> list[str]:
         "cut_das_device_noalloc",
         "cut_ncc3d_search", "cut_envelope", "cut_rf_hilbert_format", "cut_das_device_format",
         "cut_beamform_batch", "cut_set_autotune", "cut_batch_launch_shape",
+        "cut_set_fft_tile", "cut_tuning_save", "cut_tuning_load",
         "cut_pnet_param_count", "cut_pnet_set_precision", "cut_pnet_set_input_mode", "cut_pnet_gather", "cut_pnet_forward", "cut_pnet_loss_grad", "cut_pnet_adam",
         "cut_pnet_infer", "cut_motion3d_learned",
         "cut_synth_rf",

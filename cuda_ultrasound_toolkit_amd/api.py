@@ -282,6 +282,19 @@ def set_autotune(on: bool) -> bool:
     return bool(L.load().cut_set_autotune(int(bool(on))))
 
 
+def set_fft_tile(values_per_thread: int) -> bool:
+    """Hilbert FFT tile: 0 tuned, 8 or 16 values per thread."""
+    return bool(L.load().cut_set_fft_tile(values_per_thread))
+
+
+def tuning_save(path) -> bool:
+    return bool(L.load().cut_tuning_save(str(path).encode()))
+
+
+def tuning_load(path) -> bool:
+    return bool(L.load().cut_tuning_load(str(path).encode()))
+
+
 def batch_launch_shape() -> tuple[int, int]:
     g, o = C.c_int(), C.c_int()
     L.load().cut_batch_launch_shape(C.byref(g), C.byref(o))

@@ -152,5 +152,5 @@ def build_sanitized(verbose: bool = True) -> Path:
 
 if __name__ == "__main__":
     build(incremental="--incremental" in sys.argv)
-    if "--sanitize" in sys.argv:
+    if sanitizer_runtime() is not None and "--no-sanitize" not in sys.argv:
         build_sanitized()

@@ -145,6 +145,15 @@ bool cut_set_das_options(int cubic, int cf_hercules, int iq_format)
     return true;
 }
 
+bool cut_set_fft_tile(int values_per_thread)
+{
+    if (values_per_thread != 0 && values_per_thread != 8 && values_per_thread != 16)
+        CUT_FAIL("FFT tile must be 0 (tuned), 8 or 16 values per thread");
+    std::lock_guard<std::mutex> lock(cut::api_mutex());
+    cut::options().fft_vpt = values_per_thread;
+    return true;
+}
+
 bool cut_beamform_device(const void* d_raw, size_t raw_bytes, void* d_volume, size_t volume_bytes,
                          const CudaBeamformerParameters* bp)
 {

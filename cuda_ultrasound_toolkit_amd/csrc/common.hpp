@@ -111,28 +111,32 @@ bool launch_filter_spectrum(const float* d_h, int L, int S, float2* d_F, hipStre
 // power of two chosen so int16-derived data cannot overflow), bf16x2.
 enum IqFormat { IQ_F32 = 0, IQ_F16 = 1, IQ_BF16 = 2 };
 inline size_t iq_sample_bytes(int fmt) { return fmt == IQ_F32 ? 8 : 4; }
+// vpt: the FFT tile, values per thread in the LDS transforms (8 or 16; see
+// RfPipeline::hilbert for how it is chosen).  Both compute the same transform
+// with a different pass structure (rounding differs at the 1e-7 level).
 bool launch_hilbert(const HilbertInput& in, int rows, int S, const float2* d_tw, const float2* d_F, void* out,
-                    int out_fmt, float out_scale, hipStream_t st);
+                    int out_fmt, float out_scale, int vpt, hipStream_t st);
 bool hilbert_supported(int S);  // 1 <= S <= 4096, powers of two up to 65536, others up to 32768
 bool hilbert_pow2(int S);       // powers of two 16..4096 take the radix-16 kernel
-// Tables of the chirp-z (Bluestein) path for other lengths: M = 2^bluestein_log2(S) >= 2S - 1.
-struct BluesteinTables {
-    const float2* chirp;  // S entries
-    const float2* V;      // M entries
-    const float2* Wt;     // M entries
-    const float2* tw;     // M-point twiddles
-};
-int bluestein_log2(int S);
-bool launch_bluestein_tables(int S, float2* chirp, float2* V, float2* Wt, float2* tw, hipStream_t st);
-bool launch_hilbert_bluestein(const HilbertInput& in, int rows, int S, const BluesteinTables& bt, const float2* d_F,
-                              void* out, int ofmt, float oscale, hipStream_t st);
+// Other lengths: the operator as a cyclic convolution, evaluated with
+// M = 2^conv_log2(S) >= 2S - 1 point FFTs of the zero-padded rows; H = the
+// M-point spectrum of the operator's kernel (with weights, filter and 1/M),
+// built in FP64 by launch_conv_tables (d_tmp: conv_tables_tmp_bytes(S)); d_h,
+// L: the match filter taps (L = 0: none).
+int conv_log2(int S);
+bool hilbert_conv(int S);     // the length takes the convolution form
+int hilbert_fft_len(int S);   // transform length: S, or 2^conv_log2(S) for the convolution form
+size_t conv_tables_tmp_bytes(int S);
+bool launch_conv_tables(const float* d_h, int L, int S, void* d_tmp, float2* d_H, hipStream_t st);
+bool launch_hilbert_conv(const HilbertInput& in, int rows, int S, const float2* d_tw, const float2* d_H, void* out,
+                         int ofmt, float oscale, int vpt, hipStream_t st);
 // Rows of more than 4096 samples: N-point transforms (N = S for powers of
-// two, else the chirp-z length 2^bluestein_log2(S)) as N / 4096 x 4096 steps
+// two, else the convolution length 2^conv_log2(S)) as N / 4096 x 4096 steps
 // through HBM, N <= HILBERT_LONG_MAX_N.  Tables: twN = N-point twiddles, tw2 =
-// 4096-point twiddles, and for the chirp-z form chirp (S), V and Wt (N) as
-// BluesteinTables.  The steps run in place in the output when it is complex
-// f32 and S is a power of two; otherwise in a workspace of ws_rows rows of N
-// complex values (hilbert_long_chunk_rows), rows taken in chunks of ws_rows.
+// 4096-point twiddles, H = the convolution spectrum (other lengths).  The
+// steps run in place in the output when it is complex f32 and S is a power
+// of two; otherwise in a workspace of ws_rows rows of N complex values
+// (hilbert_long_chunk_rows), rows taken in chunks of ws_rows.
 constexpr int HILBERT_LONG_MAX_N = 65536;
 bool hilbert_long(int S);       // S > 4096
 int hilbert_long_n(int S);
@@ -141,9 +145,7 @@ bool hilbert_long_in_place(int S, int ofmt);
 struct LongHilbertTables {
     const float2* twN;
     const float2* tw2;
-    const float2* chirp;
-    const float2* V;
-    const float2* Wt;
+    const float2* H;
 };
 bool launch_hilbert_long(const HilbertInput& in, int rows, int S, const LongHilbertTables& t, const float2* d_F,
                          void* out, int ofmt, float oscale, float2* ws, size_t ws_rows, hipStream_t st);

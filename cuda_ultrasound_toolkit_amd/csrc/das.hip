@@ -19,6 +19,7 @@
 // is too long or does not fit the pool falls back to direct global reads for
 // that channel.
 #include "common.hpp"
+#include "tuning.hpp"
 
 #include <hip/hip_fp16.h>
 
@@ -1015,19 +1016,22 @@ __global__ __launch_bounds__(1024) void tpw_kernel(const float2* __restrict__ iq
     store_voxel(vl, acc, inc, p.cw, vol);
 }
 
-// Launch tuning: the transmit split of the single-frame HERCULES launch (the
-// heuristic das_splits, or the fastest of 1, 2, 4, 8 and 16 splits) and the waves
-// per voxel group of the per-voxel kernels (vox_waves, or the fastest of 1 to
-// 16), measured the first time a shape is seen (outside stream capture).
+// Launch tuning (tuning.hpp): the transmit split of the single-frame HERCULES
+// launch (the heuristic das_splits, or the fastest of 1, 2, 4, 8 and 16
+// splits) and the waves per voxel group of the per-voxel kernels (vox_waves,
+// or the fastest of 1 to 16), measured the first time a shape is seen
+// (outside stream capture).
 namespace {
-using TuneKey = std::array<long long, 12>;
-std::mutex g_tune_mutex;
-std::map<TuneKey, int> g_tuned_splits;  // HERCULES transmit splits
-std::map<TuneKey, int> g_tuned_waves;   // waves per voxel group of the per-voxel kernels
-bool g_das_autotune = true;
 constexpr int TUNE_MAX_SPLITS = 16;
 
 int max_splits(int gt_total) { return std::max(1, std::min(TUNE_MAX_SPLITS, gt_total / 8)); }
+
+std::vector<int> pow2_upto(int n)
+{
+    std::vector<int> c;
+    for (int v = 1; v <= n; v *= 2) c.push_back(v);
+    return c;
+}
 
 // Waves per 64-voxel group: enough for ~8192 waves in flight (32 per CU),
 // at most VOX_MAX_WAVES.
@@ -1038,62 +1042,7 @@ int vox_waves(long long nvox)
     while (nw < VOX_MAX_WAVES && groups * nw < 8192) nw *= 2;
     return nw;
 }
-
-// Runs run(choice) with the choice tuned for `key`, else the default; with
-// autotuning on and the stream not capturing, the first call of a key runs
-// the default once untimed (first-launch costs), then times every candidate
-// (1, 2, 4, ... up to max_cand) twice and keeps the fastest.  Every candidate
-// computes the complete result (only the summation order differs), so the
-// last timed run's output stands.  allow_tune false: never time (no events are
-// created), use the tuned choice if there is one, else the default.
-template <class Run>
-bool tuned_run(std::map<TuneKey, int>& table, const TuneKey& key, int dflt, int max_cand, hipStream_t st, bool allow_tune,
-               Run&& run)
-{
-    int choice = dflt;
-    bool tune = false;
-    {
-        std::lock_guard<std::mutex> lock(g_tune_mutex);
-        auto it = table.find(key);
-        if (it != table.end()) choice = it->second;
-        else tune = allow_tune && g_das_autotune && max_cand > 1;
-    }
-    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-    if (!tune || hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return run(choice);
-    if (!run(dflt)) return false;
-    hipEvent_t a, b;
-    CUT_HIP_CHECK(hipEventCreate(&a));
-    CUT_HIP_CHECK(hipEventCreate(&b));
-    float best = 1e30f;
-    int best_c = dflt;
-    for (int k = 0; k < 2 * 16 && (1 << (k >> 1)) <= max_cand; ++k) {
-        const int cand = 1 << (k >> 1);
-        float ms = 0.0f;
-        const bool ok = hipEventRecord(a, st) == hipSuccess && run(cand) && hipEventRecord(b, st) == hipSuccess &&
-                        hipEventSynchronize(b) == hipSuccess && hipEventElapsedTime(&ms, a, b) == hipSuccess;
-        if (!ok) {
-            (void)hipEventDestroy(a);
-            (void)hipEventDestroy(b);
-            CUT_FAIL("DAS: launch tuning failed");
-        }
-        if (ms < best) {
-            best = ms;
-            best_c = cand;
-        }
-    }
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
-    std::lock_guard<std::mutex> lock(g_tune_mutex);
-    table[key] = best_c;
-    return true;
-}
 }  // namespace
-
-void set_das_autotune(bool on)
-{
-    std::lock_guard<std::mutex> lock(g_tune_mutex);
-    g_das_autotune = on;
-}
 
 size_t das_workspace_bytes(const DasParams& p)
 {
@@ -1102,10 +1051,7 @@ size_t das_workspace_bytes(const DasParams& p)
     const long long tiles = (long long)((p.Nx + HT_X - 1) / HT_X) * ((p.Ny + HT_Y - 1) / HT_Y) * ((p.Nz + HT_Z - 1) / HT_Z);
     const int G = p.readi_groups > 1 ? p.readi_groups : 1;
     int splits = das_splits(tiles, G * p.T);
-    {
-        std::lock_guard<std::mutex> lock(g_tune_mutex);
-        if (g_das_autotune) splits = std::max(splits, max_splits(G * p.T));
-    }
+    if (autotune_enabled()) splits = std::max(splits, max_splits(G * p.T));
     return splits > 1 ? (size_t)splits * p.Nx * p.Ny * p.Nz * sizeof(float4) : 0;
 }
 
@@ -1280,7 +1226,7 @@ bool launch_das(const void* iq, int iq_fmt, float iq_scale, const DasParams& p, 
         };
         const TuneKey key{p.Nx, p.Ny, p.Nz, p.T, p.C, p.S, G, iq_fmt, o.cubic, cf, focus, sinapo};
         const int cand = fits(max_splits(G * p.T)) ? max_splits(G * p.T) : 1;
-        if (!tuned_run(g_tuned_splits, key, dflt, cand, st, allow_tune, run)) return false;
+        if (!tuned_run(TUNE_DAS_SPLITS, key, dflt, pow2_upto(cand), st, allow_tune, run)) return false;
     } else if (p.sequence == FORCES) {
         if (p.readi_groups > 1 && p.encoding != HADAMARD) CUT_FAIL("FORCES only supports Hadamard encoding at this time.");
         if (iq_fmt != IQ_F32) CUT_FAIL("DAS: staged IQ formats are supported for HERCULES only");
@@ -1295,7 +1241,8 @@ bool launch_das(const void* iq, int iq_fmt, float iq_scale, const DasParams& p, 
             return true;
         };
         const TuneKey key{FORCES, p.Nx, p.Ny, p.Nz, p.T, p.C, p.S, p.readi_groups, o.cubic, p.apo_type, 0, 0};
-        if (!tuned_run(g_tuned_waves, key, vox_waves(nvox), VOX_MAX_WAVES, st, allow_tune, run)) return false;
+        if (!tuned_run(TUNE_VOX_WAVES, key, vox_waves(nvox), pow2_upto(VOX_MAX_WAVES), st, allow_tune, run))
+            return false;
     } else if (p.sequence == UFORCES || p.sequence == RCA_TPW) {
         if (iq_fmt != IQ_F32) CUT_FAIL("DAS: staged IQ formats are supported for HERCULES only");
         if (p.T > MAX_ACQ_COUNT) CUT_FAIL("DAS: at most " << MAX_ACQ_COUNT << " transmits");
@@ -1328,7 +1275,8 @@ bool launch_das(const void* iq, int iq_fmt, float iq_scale, const DasParams& p, 
             return true;
         };
         const TuneKey key{p.sequence, p.Nx, p.Ny, p.Nz, p.T, p.C, p.S, p.focus_type, o.cubic, 0, 0, 0};
-        if (!tuned_run(g_tuned_waves, key, vox_waves(nvox), VOX_MAX_WAVES, st, allow_tune, run)) return false;
+        if (!tuned_run(TUNE_VOX_WAVES, key, vox_waves(nvox), pow2_upto(VOX_MAX_WAVES), st, allow_tune, run))
+            return false;
     } else {
         CUT_FAIL("Beamformer: Unsupported sequence ID " << p.sequence);
     }

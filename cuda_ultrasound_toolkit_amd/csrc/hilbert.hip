@@ -7,8 +7,9 @@
 //   y = unnormalised inverse DFT_S(Y).
 //
 // Implementation: Stockham auto-sort FFTs held in LDS.  Each thread owns 16
-// complex values per pass (radix-16 passes, one radix-2/4/8 pass when log2 S
-// is not a multiple of 4), so S = 4096 takes three passes per transform.
+// (or 8: the FFT tile, tuned per length) complex values per pass (radix-16
+// passes, one radix-2/4/8 pass when log2 S is not a multiple of 4), so
+// S = 4096 takes three passes per transform.
 // Power-of-two rows are taken in pairs: one forward transform of
 // x_a + i x_b yields both spectra, then one inverse transform per row
 // (three transforms per pair instead of four).  The first forward pass reads
@@ -16,9 +17,12 @@
 // int16/f32 through the channel map, so convert+Hilbert is one kernel when no
 // decode is needed), the spectral weight and filter are applied while the
 // two spectra are separated, and the last inverse pass writes each complex
-// row straight to HBM.  Other lengths use Bluestein's method on a power-of-two
-// FFT.  LDS rows are padded by one complex per 16 so the radix-16 transposes
-// are free of bank conflicts.  Twiddles come from a table generated in FP64.
+// row straight to HBM.  Other lengths are a cyclic convolution with the
+// operator's kernel, evaluated by power-of-two FFTs of the zero-padded row
+// (hilbert_conv_kernel); rows longer than 4096 take HBM-staged four-step
+// transforms.  LDS rows are padded by one complex per 16 so the radix-16
+// transposes are free of bank conflicts.  Twiddles come from a table
+// generated in FP64.
 #include "common.hpp"
 
 #include <hip/hip_fp16.h>
@@ -172,17 +176,17 @@ __device__ __forceinline__ float load_at(const void* base, unsigned stride, unsi
     else return ((const float*)base)[idx];
 }
 
-// One Stockham pass over an S-point row held by P = S/16 threads.  The thread
-// holds 16 values: 16/R butterflies of radix R.  Load phase -> (barrier) ->
-// twiddle + DFT + store phase.
-template <int S, int R, bool INV, typename LoadF, typename StoreF>
+// One Stockham pass over an S-point row held by P = S/VPT threads.  The
+// thread holds VPT values (the FFT tile: 16 or 8): VPT/R butterflies of radix
+// R.  Load phase -> (barrier) -> twiddle + DFT + store phase.
+template <int S, int R, bool INV, int VPT, typename LoadF, typename StoreF>
 __device__ __forceinline__ void stockham_pass(int tid, int p, const float2* __restrict__ tw, LoadF load,
                                               StoreF store, bool sync_between)
 {
-    constexpr int P = S / 16;
+    constexpr int P = S / VPT;
     constexpr int NB = S / R;
-    constexpr int PER = 16 / R;
-    float2 u[16];
+    constexpr int PER = VPT / R;
+    float2 u[VPT];
 #pragma unroll
     for (int m = 0; m < PER; ++m) {
         const int i = tid + m * P;
@@ -213,39 +217,46 @@ __device__ __forceinline__ uint32_t bf16_bits(float f)
     return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
 }
 
-// All passes of one N-point transform held in LDS.  The first pass reads
-// through first_load (global memory, or LDS when first_lds), the last writes
-// through last_store; a barrier follows every pass whose output later passes
-// or transforms read.
-template <int LOGN, bool INV, typename LoadF, typename StoreF>
+// All passes of one N-point transform held in LDS: radix-VPT passes, then one
+// radix 2..VPT/2 pass when log2 N is not a multiple of log2 VPT.  The first
+// pass reads through first_load (global memory, or LDS when first_lds), the
+// last writes through last_store; a barrier follows every pass whose output
+// later passes or transforms read.
+template <int LOGN, bool INV, int VPT = 16, typename LoadF, typename StoreF>
 __device__ __forceinline__ void fft_passes(int tid, const float2* __restrict__ tw, float2* lds, LoadF first_load,
                                            bool first_lds, StoreF last_store, bool last_global)
 {
+    static_assert(VPT == 8 || VPT == 16, "FFT tile: 8 or 16 values per thread");
+    constexpr int LV = VPT == 16 ? 4 : 3;
     constexpr int N = 1 << LOGN;
-    constexpr int NP16 = LOGN / 4;
-    constexpr int RT = 1 << (LOGN % 4);
+    constexpr int NPV = LOGN / LV;
+    constexpr int RT = 1 << (LOGN % LV);
     auto load_lds = [&](int i) { return lds[lds_idx(i)]; };
     auto store_lds = [&](int i, float2 v) { lds[lds_idx(i)] = v; };
     int p = 1;
 #pragma unroll
-    for (int q = 0; q < NP16; ++q) {
-        const bool last = (q == NP16 - 1) && RT == 1;
+    for (int q = 0; q < NPV; ++q) {
+        const bool last = (q == NPV - 1) && RT == 1;
         if (q == 0) {
-            if (last) stockham_pass<N, 16, INV>(tid, p, tw, first_load, last_store, first_lds);
-            else stockham_pass<N, 16, INV>(tid, p, tw, first_load, store_lds, first_lds);
+            if (last) stockham_pass<N, VPT, INV, VPT>(tid, p, tw, first_load, last_store, first_lds);
+            else stockham_pass<N, VPT, INV, VPT>(tid, p, tw, first_load, store_lds, first_lds);
         } else {
-            if (last) stockham_pass<N, 16, INV>(tid, p, tw, load_lds, last_store, true);
-            else stockham_pass<N, 16, INV>(tid, p, tw, load_lds, store_lds, true);
+            if (last) stockham_pass<N, VPT, INV, VPT>(tid, p, tw, load_lds, last_store, true);
+            else stockham_pass<N, VPT, INV, VPT>(tid, p, tw, load_lds, store_lds, true);
         }
         if (!(last && last_global)) __syncthreads();
-        p *= 16;
+        p *= VPT;
     }
     if constexpr (RT > 1) {
-        if (NP16 == 0) stockham_pass<N, RT, INV>(tid, p, tw, first_load, last_store, first_lds);
-        else stockham_pass<N, RT, INV>(tid, p, tw, load_lds, last_store, true);
+        if (NPV == 0) stockham_pass<N, RT, INV, VPT>(tid, p, tw, first_load, last_store, first_lds);
+        else stockham_pass<N, RT, INV, VPT>(tid, p, tw, load_lds, last_store, true);
         if (!last_global) __syncthreads();
     }
 }
+
+// Threads of a workgroup for rows of 2^LOG points at VPT values per thread:
+// at least 256 (several rows of short transforms share a workgroup).
+constexpr int fft_block(int LOG, int VPT) { return (1 << LOG) / VPT > 256 ? (1 << LOG) / VPT : 256; }
 
 // Two rows a, b per transform (power-of-two S): one forward FFT of the
 // complex row z = x_a + i x_b gives both real rows' spectra,
@@ -258,14 +269,16 @@ __device__ __forceinline__ void fft_passes(int tid, const float2* __restrict__ t
 // barrier of its own.  A lone last row (odd row count) pairs with zeros.
 // Rounding of the shared transform scales with the larger row of the pair
 // (FP32, ~1e-7 of its magnitude).
-template <int LOGS, int KIND, bool FILTER>
-__global__ __launch_bounds__(256) void hilbert_pair_kernel(HilbertDev in, int rows, const float2* __restrict__ tw,
-                                                           const float2* __restrict__ F, void* __restrict__ out,
-                                                           int ofmt, float oscale)
+template <int LOGS, int KIND, bool FILTER, int VPT>
+__global__ __launch_bounds__(fft_block(LOGS, VPT)) void hilbert_pair_kernel(HilbertDev in, int rows,
+                                                                            const float2* __restrict__ tw,
+                                                                            const float2* __restrict__ F,
+                                                                            void* __restrict__ out, int ofmt,
+                                                                            float oscale)
 {
     constexpr int S = 1 << LOGS;
     constexpr int H = S / 2;
-    constexpr int P = S / 16;
+    constexpr int P = S / VPT;
     constexpr int RPW = P >= 256 ? 1 : 256 / P;     // row pairs per workgroup
     constexpr int PADDED = S + S / 16;
     constexpr int PADDED2 = H + H / 16 + 1;          // lds_idx(H) + 1
@@ -289,7 +302,7 @@ __global__ __launch_bounds__(256) void hilbert_pair_kernel(HilbertDev in, int ro
                            act_b ? load_at<KIND>(base_b, istride, (unsigned)idx) : 0.0f);
     };
     auto store_lds = [&](int idx, float2 v) { lds[lds_idx(idx)] = v; };
-    fft_passes<LOGS, false>(tid, tw, lds, load_pair, false, store_lds, false);
+    fft_passes<LOGS, false, VPT>(tid, tw, lds, load_pair, false, store_lds, false);
 
     // split into the two one-sided spectra, weight (w_k / 2) and filter
     const float half_w = 0.75f / (float)S;
@@ -333,33 +346,32 @@ __global__ __launch_bounds__(256) void hilbert_pair_kernel(HilbertDev in, int ro
     // workgroups per CU, without; 162 and three with).
     const float2* tw_a = tw;
     __asm__ volatile("" : "+s"(tw_a));
-    fft_passes<LOGS, true>(tid, tw_a, lds, load_ya, true, store_a, true);
+    fft_passes<LOGS, true, VPT>(tid, tw_a, lds, load_ya, true, store_a, true);
     __syncthreads();  // the last pass of a read the row buffer that b's first pass writes
     const float2* tw_b = tw;
     __asm__ volatile("" : "+s"(tw_b));
-    fft_passes<LOGS, true>(tid, tw_b, lds, load_yb, false, store_b, true);
+    fft_passes<LOGS, true, VPT>(tid, tw_b, lds, load_yb, false, store_b, true);
 }
 
-// Hilbert transform of rows of any length S (not a power of two) by
-// Bluestein's chirp-z method: with w[n] = exp(-i pi n^2 / S),
-//   DFT_S(x)[k]  = w[k] * sum_n (x[n] w[n]) conj(w)[k - n],
-//   IDFT_S(Y)[n] = conj(w[n]) * sum_k (Y[k] conj(w[k])) w[n - k],
-// each convolution evaluated as a cyclic one of length M = 2^LOGM >= 2S - 1
-// by FFTs held in LDS (V = FFT_M of the even extension of conj(w), Wt = the
-// same of w, both divided by M).  Four M-point transforms per row; the
-// spectral weight and filter are applied between them as for the
-// power-of-two kernel.
-template <int LOGM, int KIND, bool FILTER>
-__global__ __launch_bounds__(512) void hilbert_bluestein_kernel(HilbertDev in, int rows, int S,
-                                                                const float2* __restrict__ tw,
-                                                                const float2* __restrict__ chirp,
-                                                                const float2* __restrict__ V,
-                                                                const float2* __restrict__ Wt,
-                                                                const float2* __restrict__ F, void* __restrict__ out,
-                                                                int ofmt, float oscale)
+// Hilbert transform of rows of any length S <= 4096 that is not a power of
+// two.  The reference's operator, y = IDFT_S(G . DFT_S(x)) with G[k] = w_k F[k]
+// for k <= S/2 and 0 above (unnormalised inverse), is a cyclic convolution of
+// length S with g = IDFT_S(G): y[n] = sum_m x[m] g[(n - m) mod S].  With the row
+// zero-padded to M = 2^conv_log2(S) >= 2S - 1 and h the kernel laid out over
+// the offsets -(S-1)..S-1 (h[j mod M] = g[j mod S]), the cyclic convolution of
+// length M equals it for n < S (no offset wraps around), so
+//   y[n] = IFFT_M(FFT_M(x) . H)[n],  H = FFT_M(h) / M,
+// two M-point transforms per row held in LDS, with H computed once per length
+// and filter in FP64 (conv_tables).
+template <int LOGM, int KIND, int VPT>
+__global__ __launch_bounds__(fft_block(LOGM, VPT)) void hilbert_conv_kernel(HilbertDev in, int rows, int S,
+                                                                           const float2* __restrict__ tw,
+                                                                           const float2* __restrict__ H,
+                                                                           void* __restrict__ out, int ofmt,
+                                                                           float oscale)
 {
     constexpr int M = 1 << LOGM;
-    constexpr int P = M / 16;
+    constexpr int P = M / VPT;
     constexpr int RPW = P >= 256 ? 1 : 256 / P;
     constexpr int PADDED = M + M / 16;
     __shared__ float2 buf[RPW * PADDED];
@@ -370,109 +382,106 @@ __global__ __launch_bounds__(512) void hilbert_bluestein_kernel(HilbertDev in, i
     const bool active = row < rows;
     float2* lds = buf + rsub * PADDED;
     const int rrow = active ? row : 0;
-    const float scale = 1.5f / (float)S;
-    const int half = S / 2;
+    const void* base = row_base<KIND>(in, rrow, S);
+    const unsigned istride = (unsigned)in.stride;
 
-    auto load_a = [&](int n) {
-        if (n >= S || !active) return make_float2(0.0f, 0.0f);
-        const float x = load_in<KIND>(in, rrow, S, n);
-        const float2 c = chirp[n];
-        return make_float2(x * c.x, x * c.y);
+    auto load_x = [&](int n) {
+        return make_float2(active && n < S ? load_at<KIND>(base, istride, (unsigned)n) : 0.0f, 0.0f);
     };
     auto load_lds = [&](int i) { return lds[lds_idx(i)]; };
-    auto store_V = [&](int k, float2 v) { lds[lds_idx(k)] = cmul(v, V[k]); };
-    auto store_W = [&](int k, float2 v) { lds[lds_idx(k)] = cmul(v, Wt[k]); };
-    // X[k] = w[k] c[k]; Y = weight * X * F for k <= S/2, 0 above; next input Y[k] conj(w[k])
-    auto store_mid = [&](int k, float2 v) {
-        float2 r = make_float2(0.0f, 0.0f);
-        if (k <= half) {
-            const float2 c = chirp[k];
-            float2 X = cmul(v, c);
-            const float wk = k == 0 ? scale * 0.5f : scale;
-            X = make_float2(X.x * wk, X.y * wk);
-            if constexpr (FILTER) X = cmul(X, F[k]);
-            r = cmul(X, make_float2(c.x, -c.y));
-        }
-        lds[lds_idx(k)] = r;
-    };
+    auto store_H = [&](int k, float2 v) { lds[lds_idx(k)] = cmul(v, H[k]); };
     const size_t obase = (size_t)rrow * S;
     auto store_out = [&](int n, float2 v) {
         if (!active || n >= S) return;
-        const float2 c = chirp[n];
-        const float2 y = cmul(v, make_float2(c.x, -c.y));
         if (ofmt == IQ_F32) {
-            static_cast<float2*>(out)[obase + n] = y;
+            static_cast<float2*>(out)[obase + n] = v;
         } else if (ofmt == IQ_F16) {
-            static_cast<__half2*>(out)[obase + n] = __floats2half2_rn(y.x * oscale, y.y * oscale);
+            static_cast<__half2*>(out)[obase + n] = __floats2half2_rn(v.x * oscale, v.y * oscale);
         } else {
-            static_cast<uint32_t*>(out)[obase + n] = bf16_bits(y.x) | (bf16_bits(y.y) << 16);
+            static_cast<uint32_t*>(out)[obase + n] = bf16_bits(v.x) | (bf16_bits(v.y) << 16);
         }
     };
-    // each transform through its own opaque copy of the twiddle pointer (see
-    // hilbert_pair_kernel): no twiddle registers live from one to the next
-    const float2* tw1 = tw;
-    fft_passes<LOGM, false>(tid, tw1, lds, load_a, false, store_V, false);
+    fft_passes<LOGM, false, VPT>(tid, tw, lds, load_x, false, store_H, false);
+    // the inverse reads the twiddles through its own opaque copy of the
+    // pointer (see hilbert_pair_kernel)
     const float2* tw2 = tw;
     __asm__ volatile("" : "+s"(tw2));
-    fft_passes<LOGM, true>(tid, tw2, lds, load_lds, true, store_mid, false);
-    const float2* tw3 = tw;
-    __asm__ volatile("" : "+s"(tw3));
-    fft_passes<LOGM, false>(tid, tw3, lds, load_lds, true, store_W, false);
-    const float2* tw4 = tw;
-    __asm__ volatile("" : "+s"(tw4));
-    fft_passes<LOGM, true>(tid, tw4, lds, load_lds, true, store_out, true);
+    fft_passes<LOGM, true, VPT>(tid, tw2, lds, load_lds, true, store_out, true);
 }
 
-// Bluestein tables in FP64: chirp[n] = exp(-i pi (n^2 mod 2S) / S), n < S.
-__global__ void chirp_kernel(float2* chirp, int S)
+// Kernel spectrum of the convolution form, in FP64 (exact integer angle
+// reduction):  G[q] = w_q F[q] for q <= S/2 (F = DFT_S of the taps, 1 without
+// a filter; w_0 = 0.75/S, w_q = 1.5/S),  g[n] = sum_q G[q] exp(+2 pi i q n / S),
+//   H[k] = (1/M) sum_{j=-(S-1)}^{S-1} g[j mod S] exp(-2 pi i j k / M).
+__global__ void conv_G_kernel(const float* __restrict__ h, int L, int S, double2* __restrict__ G)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q > S / 2) return;
+    double re = 1.0, im = 0.0;
+    if (L > 0) {
+        re = 0.0;
+        for (int n = 0; n < L; ++n) {
+            const long long e = ((long long)q * n) % S;
+            double sn, cs;
+            sincospi(2.0 * (double)e / (double)S, &sn, &cs);
+            re += (double)h[n] * cs;
+            im -= (double)h[n] * sn;
+        }
+    }
+    const double w = (q == 0 ? 0.75 : 1.5) / (double)S;
+    G[q] = make_double2(re * w, im * w);
+}
+
+__global__ void conv_g_kernel(const double2* __restrict__ G, int S, double2* __restrict__ g)
 {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= S) return;
-    const long long q = ((long long)n * n) % (2LL * S);
-    double sn, cs;
-    sincospi((double)q / (double)S, &sn, &cs);
-    chirp[n] = make_float2((float)cs, (float)-sn);
+    double re = 0.0, im = 0.0;
+    for (int q = 0; q <= S / 2; ++q) {
+        const long long e = ((long long)q * n) % S;
+        double sn, cs;
+        sincospi(2.0 * (double)e / (double)S, &sn, &cs);
+        re += G[q].x * cs - G[q].y * sn;
+        im += G[q].x * sn + G[q].y * cs;
+    }
+    g[n] = make_double2(re, im);
 }
 
-// out[k] = (1/M) sum_m u[m] exp(-2 pi i m k / M) over the even extension of
-// u[m] = exp(sgn i pi m^2 / S) (m in (-S, S), index m mod M); sgn = +1 gives
-// the spectrum of conj(chirp), -1 that of chirp.  The terms of m and -m share
-// u[m], so the sum is 1 + 2 sum_{m=1}^{S-1} u[m] cos(2 pi (m k mod M) / M);
-// m^2 mod 2S is carried from m to m + 1 in exact integers.
-__global__ void chirp_spectrum_kernel(float2* out, int S, int M, int sgn)
+__global__ void conv_H_kernel(const double2* __restrict__ g, int S, int M, float2* __restrict__ H)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= M) return;
-    double re = 1.0, im = 0.0;
-    long long q = 0;  // m^2 mod 2S
-    for (int m = 1; m < S; ++m) {
-        q += 2LL * m - 1;
-        if (q >= 2LL * S) q -= 2LL * S;
-        const unsigned e = ((unsigned)m * (unsigned)k) & (unsigned)(M - 1);
+    double re = 0.0, im = 0.0;
+    for (int j = -(S - 1); j < S; ++j) {
+        const double2 v = g[j < 0 ? j + S : j];
+        const unsigned long long e = ((unsigned long long)(j < 0 ? j + M : j) * (unsigned long long)k) & (M - 1);
         double sn, cs;
-        sincospi(sgn * (double)q / (double)S, &sn, &cs);
-        const double c = 2.0 * cospi(2.0 * (double)e / (double)M);
-        re += c * cs;
-        im += c * sn;
+        sincospi(2.0 * (double)e / (double)M, &sn, &cs);
+        re += v.x * cs + v.y * sn;
+        im += v.y * cs - v.x * sn;
     }
-    out[k] = make_float2((float)(re / M), (float)(im / M));
+    H[k] = make_float2((float)(re / M), (float)(im / M));
 }
 
-int bluestein_log2(int S)
+int conv_log2(int S)
 {
     int lg = 4;
     while ((1 << lg) < 2 * S - 1) ++lg;
     return lg;
 }
 
-bool launch_bluestein_tables(int S, float2* chirp, float2* V, float2* Wt, float2* tw, hipStream_t st)
+size_t conv_tables_tmp_bytes(int S) { return (size_t)(S / 2 + 1 + S) * sizeof(double2); }
+
+bool launch_conv_tables(const float* d_h, int L, int S, void* d_tmp, float2* d_H, hipStream_t st)
 {
-    const int M = 1 << bluestein_log2(S);
-    chirp_kernel<<<(S + 255) / 256, 256, 0, st>>>(chirp, S);
-    chirp_spectrum_kernel<<<(M + 127) / 128, 128, 0, st>>>(V, S, M, +1);
-    chirp_spectrum_kernel<<<(M + 127) / 128, 128, 0, st>>>(Wt, S, M, -1);
+    const int M = 1 << conv_log2(S);
+    double2* G = static_cast<double2*>(d_tmp);
+    double2* g = G + (S / 2 + 1);
+    conv_G_kernel<<<(S / 2 + 1 + 127) / 128, 128, 0, st>>>(d_h, L, S, G);
+    conv_g_kernel<<<(S + 127) / 128, 128, 0, st>>>(G, S, g);
+    conv_H_kernel<<<(M + 127) / 128, 128, 0, st>>>(g, S, M, d_H);
     CUT_HIP_CHECK(hipGetLastError());
-    return launch_twiddles(tw, M, st);
+    return true;
 }
 
 // Twiddle table tw[m] = exp(-2 pi i m / S), generated in FP64.
@@ -518,64 +527,78 @@ bool launch_filter_spectrum(const float* d_h, int L, int S, float2* d_F, hipStre
 
 bool hilbert_pow2(int S) { return S >= 16 && (S & (S - 1)) == 0 && S <= 4096; }
 bool hilbert_long(int S) { return S > 4096; }
-int hilbert_long_n(int S) { return ((S & (S - 1)) == 0) ? S : 1 << bluestein_log2(S); }
+int hilbert_long_n(int S) { return ((S & (S - 1)) == 0) ? S : 1 << conv_log2(S); }
+// Lengths that take the convolution form: every supported length except the
+// powers of two 16..4096 (pair kernel) and the powers of two beyond 4096
+// (long path with the spectral weight); so 1, 2, 4 and 8 take it too.
+bool hilbert_conv(int S)
+{
+    return hilbert_supported(S) && !hilbert_pow2(S) && !(hilbert_long(S) && (S & (S - 1)) == 0);
+}
+int hilbert_fft_len(int S) { return hilbert_conv(S) ? 1 << conv_log2(S) : S; }
 bool hilbert_supported(int S) { return S >= 1 && (S <= 4096 || hilbert_long_n(S) <= HILBERT_LONG_MAX_N); }
 
-template <int LOGM>
-static bool launch_bluestein_m(const HilbertInput& in, int rows, int S, const BluesteinTables& bt, const float2* F,
-                               void* out, int ofmt, float oscale, hipStream_t st)
+template <int LOGM, int VPT>
+static bool launch_conv_m(const HilbertInput& in, int rows, int S, const float2* tw, const float2* H, void* out,
+                          int ofmt, float oscale, hipStream_t st)
 {
     constexpr int M = 1 << LOGM;
-    constexpr int P = M / 16;
+    constexpr int P = M / VPT;
     constexpr int RPW = P >= 256 ? 1 : 256 / P;
     HilbertDev d{in.ptr, in.stride, in.map, in.R0, in.C};
     dim3 grid((rows + RPW - 1) / RPW);
     dim3 block(P * RPW);
-#define CUT_BLU(K, FL)                                                                                              \
-    hilbert_bluestein_kernel<LOGM, K, FL><<<grid, block, 0, st>>>(d, rows, S, bt.tw, bt.chirp, bt.V, bt.Wt, F, out, \
-                                                                   ofmt, oscale)
-    if (F) {
-        if (in.kind == 0) CUT_BLU(0, true); else if (in.kind == 1) CUT_BLU(1, true); else CUT_BLU(2, true);
-    } else {
-        if (in.kind == 0) CUT_BLU(0, false); else if (in.kind == 1) CUT_BLU(1, false); else CUT_BLU(2, false);
-    }
-#undef CUT_BLU
+#define CUT_CONV(K) hilbert_conv_kernel<LOGM, K, VPT><<<grid, block, 0, st>>>(d, rows, S, tw, H, out, ofmt, oscale)
+    if (in.kind == 0) CUT_CONV(0); else if (in.kind == 1) CUT_CONV(1); else CUT_CONV(2);
+#undef CUT_CONV
     CUT_HIP_CHECK(hipGetLastError());
     return true;
 }
 
-bool launch_hilbert_bluestein(const HilbertInput& in, int rows, int S, const BluesteinTables& bt, const float2* d_F,
-                              void* out, int ofmt, float oscale, hipStream_t st)
+template <int VPT>
+static bool launch_conv_vpt(const HilbertInput& in, int rows, int S, const float2* tw, const float2* H, void* out,
+                            int ofmt, float oscale, hipStream_t st)
 {
-    if (rows <= 0) return true;
-    if (ofmt != IQ_F32 && ofmt != IQ_F16 && ofmt != IQ_BF16) CUT_FAIL("Hilbert: unknown output format " << ofmt);
-    switch (bluestein_log2(S)) {
-        case 4: return launch_bluestein_m<4>(in, rows, S, bt, d_F, out, ofmt, oscale, st);
-        case 5: return launch_bluestein_m<5>(in, rows, S, bt, d_F, out, ofmt, oscale, st);
-        case 6: return launch_bluestein_m<6>(in, rows, S, bt, d_F, out, ofmt, oscale, st);
-        case 7: return launch_bluestein_m<7>(in, rows, S, bt, d_F, out, ofmt, oscale, st);
-        case 8: return launch_bluestein_m<8>(in, rows, S, bt, d_F, out, ofmt, oscale, st);
-        case 9: return launch_bluestein_m<9>(in, rows, S, bt, d_F, out, ofmt, oscale, st);
-        case 10: return launch_bluestein_m<10>(in, rows, S, bt, d_F, out, ofmt, oscale, st);
-        case 11: return launch_bluestein_m<11>(in, rows, S, bt, d_F, out, ofmt, oscale, st);
-        case 12: return launch_bluestein_m<12>(in, rows, S, bt, d_F, out, ofmt, oscale, st);
-        case 13: return launch_bluestein_m<13>(in, rows, S, bt, d_F, out, ofmt, oscale, st);
-        default: CUT_FAIL("Hilbert: sample count " << S << " not supported (1..4096)");
+    switch (conv_log2(S)) {
+        case 4: return launch_conv_m<4, VPT>(in, rows, S, tw, H, out, ofmt, oscale, st);
+        case 5: return launch_conv_m<5, VPT>(in, rows, S, tw, H, out, ofmt, oscale, st);
+        case 6: return launch_conv_m<6, VPT>(in, rows, S, tw, H, out, ofmt, oscale, st);
+        case 7: return launch_conv_m<7, VPT>(in, rows, S, tw, H, out, ofmt, oscale, st);
+        case 8: return launch_conv_m<8, VPT>(in, rows, S, tw, H, out, ofmt, oscale, st);
+        case 9: return launch_conv_m<9, VPT>(in, rows, S, tw, H, out, ofmt, oscale, st);
+        case 10: return launch_conv_m<10, VPT>(in, rows, S, tw, H, out, ofmt, oscale, st);
+        case 11: return launch_conv_m<11, VPT>(in, rows, S, tw, H, out, ofmt, oscale, st);
+        case 12: return launch_conv_m<12, VPT>(in, rows, S, tw, H, out, ofmt, oscale, st);
+        case 13: return launch_conv_m<13, VPT>(in, rows, S, tw, H, out, ofmt, oscale, st);
+        default: CUT_FAIL("Hilbert: sample count " << S << " not supported by the LDS transforms (1..4096)");
     }
 }
 
-template <int LOGS>
+bool launch_hilbert_conv(const HilbertInput& in, int rows, int S, const float2* d_tw, const float2* d_H, void* out,
+                         int ofmt, float oscale, int vpt, hipStream_t st)
+{
+    if (rows <= 0) return true;
+    if (!hilbert_conv(S) || hilbert_long(S)) CUT_FAIL("Hilbert: " << S << " samples do not take the convolution kernel");
+    if (!d_tw || !d_H) CUT_FAIL("Hilbert: convolution tables of " << S << " samples missing");
+    if (ofmt != IQ_F32 && ofmt != IQ_F16 && ofmt != IQ_BF16) CUT_FAIL("Hilbert: unknown output format " << ofmt);
+    if (vpt == 8) return launch_conv_vpt<8>(in, rows, S, d_tw, d_H, out, ofmt, oscale, st);
+    if (vpt == 16) return launch_conv_vpt<16>(in, rows, S, d_tw, d_H, out, ofmt, oscale, st);
+    CUT_FAIL("Hilbert: FFT tile of " << vpt << " values per thread (8 or 16)");
+}
+
+template <int LOGS, int VPT>
 static bool launch_hilbert_s(const HilbertInput& in, int rows, const float2* tw, const float2* F, void* out, int ofmt,
                              float oscale, hipStream_t st)
 {
     constexpr int S = 1 << LOGS;
-    constexpr int P = S / 16;
+    constexpr int P = S / VPT;
     constexpr int RPW = P >= 256 ? 1 : 256 / P;     // row pairs per workgroup
     HilbertDev d{in.ptr, in.stride, in.map, in.R0, in.C};
     const int pairs = (rows + 1) / 2;
     dim3 grid((pairs + RPW - 1) / RPW);
     dim3 block(P * RPW);
-#define CUT_HIL(K, FL) hilbert_pair_kernel<LOGS, K, FL><<<grid, block, 0, st>>>(d, rows, tw, F, out, ofmt, oscale)
+#define CUT_HIL(K, FL) \
+    hilbert_pair_kernel<LOGS, K, FL, VPT><<<grid, block, 0, st>>>(d, rows, tw, F, out, ofmt, oscale)
     if (F) {
         if (in.kind == 0) CUT_HIL(0, true); else if (in.kind == 1) CUT_HIL(1, true); else CUT_HIL(2, true);
     } else {
@@ -586,49 +609,58 @@ static bool launch_hilbert_s(const HilbertInput& in, int rows, const float2* tw,
     return true;
 }
 
-bool launch_hilbert(const HilbertInput& in, int rows, int S, const float2* d_tw, const float2* d_F, void* out,
-                    int ofmt, float oscale, hipStream_t st)
+template <int VPT>
+static bool launch_hilbert_vpt(const HilbertInput& in, int rows, int S, const float2* d_tw, const float2* d_F,
+                               void* out, int ofmt, float oscale, hipStream_t st)
 {
-    if (rows <= 0) return true;
-    if (ofmt != IQ_F32 && ofmt != IQ_F16 && ofmt != IQ_BF16) CUT_FAIL("Hilbert: unknown output format " << ofmt);
     switch (S) {
-        case 16: return launch_hilbert_s<4>(in, rows, d_tw, d_F, out, ofmt, oscale, st);
-        case 32: return launch_hilbert_s<5>(in, rows, d_tw, d_F, out, ofmt, oscale, st);
-        case 64: return launch_hilbert_s<6>(in, rows, d_tw, d_F, out, ofmt, oscale, st);
-        case 128: return launch_hilbert_s<7>(in, rows, d_tw, d_F, out, ofmt, oscale, st);
-        case 256: return launch_hilbert_s<8>(in, rows, d_tw, d_F, out, ofmt, oscale, st);
-        case 512: return launch_hilbert_s<9>(in, rows, d_tw, d_F, out, ofmt, oscale, st);
-        case 1024: return launch_hilbert_s<10>(in, rows, d_tw, d_F, out, ofmt, oscale, st);
-        case 2048: return launch_hilbert_s<11>(in, rows, d_tw, d_F, out, ofmt, oscale, st);
-        case 4096: return launch_hilbert_s<12>(in, rows, d_tw, d_F, out, ofmt, oscale, st);
+        case 16: return launch_hilbert_s<4, VPT>(in, rows, d_tw, d_F, out, ofmt, oscale, st);
+        case 32: return launch_hilbert_s<5, VPT>(in, rows, d_tw, d_F, out, ofmt, oscale, st);
+        case 64: return launch_hilbert_s<6, VPT>(in, rows, d_tw, d_F, out, ofmt, oscale, st);
+        case 128: return launch_hilbert_s<7, VPT>(in, rows, d_tw, d_F, out, ofmt, oscale, st);
+        case 256: return launch_hilbert_s<8, VPT>(in, rows, d_tw, d_F, out, ofmt, oscale, st);
+        case 512: return launch_hilbert_s<9, VPT>(in, rows, d_tw, d_F, out, ofmt, oscale, st);
+        case 1024: return launch_hilbert_s<10, VPT>(in, rows, d_tw, d_F, out, ofmt, oscale, st);
+        case 2048: return launch_hilbert_s<11, VPT>(in, rows, d_tw, d_F, out, ofmt, oscale, st);
+        case 4096: return launch_hilbert_s<12, VPT>(in, rows, d_tw, d_F, out, ofmt, oscale, st);
         default: CUT_FAIL("Hilbert: sample count " << S << " not supported (powers of two 16..4096)");
     }
+}
+
+bool launch_hilbert(const HilbertInput& in, int rows, int S, const float2* d_tw, const float2* d_F, void* out,
+                    int ofmt, float oscale, int vpt, hipStream_t st)
+{
+    if (rows <= 0) return true;
+    if (!hilbert_pow2(S)) CUT_FAIL("Hilbert: " << S << " samples do not take the row-pair kernel");
+    if (!d_tw) CUT_FAIL("Hilbert: twiddle table of " << S << " samples missing");
+    if (ofmt != IQ_F32 && ofmt != IQ_F16 && ofmt != IQ_BF16) CUT_FAIL("Hilbert: unknown output format " << ofmt);
+    if (vpt == 8) return launch_hilbert_vpt<8>(in, rows, S, d_tw, d_F, out, ofmt, oscale, st);
+    if (vpt == 16) return launch_hilbert_vpt<16>(in, rows, S, d_tw, d_F, out, ofmt, oscale, st);
+    CUT_FAIL("Hilbert: FFT tile of " << vpt << " values per thread (8 or 16)");
 }
 
 
 // ---------------------------------------------------------------------------
 // Rows longer than 4096 samples.  The N-point transforms (N = S for powers of
-// two, else the chirp-z length M = 2^ceil(log2(2S - 1))) are split as
-// N = N1 x 4096, N1 = N / 4096 <= 16, in Bailey's four-step form with the data
-// in HBM between the steps: with n = n2 + 4096 n1 and k = k1 + N1 k2,
+// two, else the convolution length M = 2^conv_log2(S) of hilbert_conv_kernel)
+// are split as N = N1 x 4096, N1 = N / 4096 <= 16, in Bailey's four-step form
+// with the data in HBM between the steps: with n = n2 + 4096 n1 and
+// k = k1 + N1 k2,
 //   X[k1 + N1 k2] = sum_n2 W_4096^(n2 k2) [W_N^(n2 k1) sum_n1 x[n2 + 4096 n1] W_N1^(n1 k1)].
 // long_fwd_pass_kernel does the inner length-N1 DFTs and the twiddles
 // (thread = one n2 of one row, loads coalesced across n2) into B[k1][n2];
 // long_row_kernel transforms each 4096-point segment B[k1][.] in LDS,
 // multiplies the spectrum (index k = k1 + N1 k2) by the spectral weight and
-// filter (or the chirp-z kernel), and transforms back; long_inv_pass_kernel
-// applies the conjugate twiddles and the inverse length-N1 DFTs, which land on
-// natural sample order.  Every step reads and writes the same addresses per
-// thread or workgroup, so the steps run in place in one row-major buffer.
-// Power-of-two rows: forward pass, row kernel (weight, filter), inverse pass
-// into the output.  Other lengths: the chirp-z form of hilbert_bluestein_kernel
-// with each cyclic convolution as these three steps (six launches).
+// filter (powers of two) or by the convolution kernel's spectrum H (other
+// lengths, rows zero-padded to N), and transforms back; long_inv_pass_kernel
+// applies the conjugate twiddles and the inverse length-N1 DFTs, which land
+// on natural sample order, and writes the row's first S samples.  Every step
+// reads and writes the same addresses per thread or workgroup, so the steps
+// run in place in one row-major buffer.  Three launches per chunk of rows.
 // ---------------------------------------------------------------------------
 constexpr int LN2_LOG = 12;
 constexpr int LN2 = 1 << LN2_LOG;
 
-enum { LPRE_REAL = 0, LPRE_REAL_CHIRP = 1, LPRE_BUF = 2 };
-enum { LPOST_OUT = 0, LPOST_MID = 1, LPOST_OUT_CHIRP = 2 };
 enum { LROW_HILBERT = 0, LROW_MUL = 1 };
 
 __device__ __forceinline__ void store_fmt(void* out, size_t i, float2 v, int ofmt, float oscale)
@@ -638,11 +670,11 @@ __device__ __forceinline__ void store_fmt(void* out, size_t i, float2 v, int ofm
     else static_cast<uint32_t*>(out)[i] = bf16_bits(v.x) | (bf16_bits(v.y) << 16);
 }
 
-// grid (LN2 / 256, rows of the chunk); rows row0 + blockIdx.y.
-template <int N1, int KIND, int PRE>
+// grid (LN2 / 256, rows of the chunk); rows row0 + blockIdx.y; samples n >= S
+// of the zero-padded row read as 0.
+template <int N1, int KIND>
 __global__ __launch_bounds__(256) void long_fwd_pass_kernel(HilbertDev in, int row0, int S,
-                                                            const float2* __restrict__ twN,
-                                                            const float2* __restrict__ chirp, float2* buf)
+                                                            const float2* __restrict__ twN, float2* buf)
 {
     constexpr int N = N1 * LN2;
     const int n2 = blockIdx.x * 256 + threadIdx.x;
@@ -651,17 +683,7 @@ __global__ __launch_bounds__(256) void long_fwd_pass_kernel(HilbertDev in, int r
 #pragma unroll
     for (int n1 = 0; n1 < N1; ++n1) {
         const int n = n2 + LN2 * n1;
-        if constexpr (PRE == LPRE_BUF) {
-            u[n1] = brow[n];
-        } else {
-            const float x = n < S ? load_in<KIND>(in, row0 + (int)blockIdx.y, S, n) : 0.0f;
-            if constexpr (PRE == LPRE_REAL_CHIRP) {
-                const float2 c = n < S ? chirp[n] : make_float2(0.0f, 0.0f);
-                u[n1] = make_float2(x * c.x, x * c.y);
-            } else {
-                u[n1] = make_float2(x, 0.0f);
-            }
-        }
+        u[n1] = make_float2(n < S ? load_in<KIND>(in, row0 + (int)blockIdx.y, S, n) : 0.0f, 0.0f);
     }
     dft<N1, false>(u);
 #pragma unroll
@@ -708,19 +730,15 @@ __global__ __launch_bounds__(256) void long_row_kernel(float2* buf, int N1, int 
     fft_passes<LN2_LOG, true>(tid, tw2, lds, load_lds, true, store_g, true);
 }
 
-// grid (LN2 / 256, rows of the chunk).  POST: LPOST_OUT writes the row
-// (samples < S) to out in format ofmt; LPOST_MID forms the chirp-z middle
-// step in place (X = chirp c, weight and filter for k <= S/2, times
-// conj(chirp), zero from S/2 + 1 on); LPOST_OUT_CHIRP writes conj(chirp) y.
-template <int N1, int POST, bool FILTER>
+// grid (LN2 / 256, rows of the chunk): samples n < S of each row to out in
+// format ofmt.
+template <int N1>
 __global__ __launch_bounds__(256) void long_inv_pass_kernel(int row0, int S, const float2* __restrict__ twN,
-                                                            const float2* __restrict__ chirp,
-                                                            const float2* __restrict__ F, float2* buf, void* out,
-                                                            int ofmt, float oscale)
+                                                            const float2* buf, void* out, int ofmt, float oscale)
 {
     constexpr int N = N1 * LN2;
     const int n2 = blockIdx.x * 256 + threadIdx.x;
-    float2* brow = buf + (size_t)blockIdx.y * N;
+    const float2* brow = buf + (size_t)blockIdx.y * N;
     float2 u[N1];
 #pragma unroll
     for (int k1 = 0; k1 < N1; ++k1) {
@@ -729,31 +747,10 @@ __global__ __launch_bounds__(256) void long_inv_pass_kernel(int row0, int S, con
     }
     dft<N1, true>(u);
     const size_t orow = (size_t)(row0 + (int)blockIdx.y) * S;
-    const float w1 = 1.5f / (float)S, w0 = 0.75f / (float)S;
-    const int half = S / 2;
 #pragma unroll
     for (int n1 = 0; n1 < N1; ++n1) {
         const int n = n2 + LN2 * n1;
-        float2 v = u[n1];
-        if constexpr (POST == LPOST_OUT) {
-            if (n < S) store_fmt(out, orow + n, v, ofmt, oscale);
-        } else if constexpr (POST == LPOST_OUT_CHIRP) {
-            if (n < S) {
-                const float2 c = chirp[n];
-                store_fmt(out, orow + n, cmul(v, make_float2(c.x, -c.y)), ofmt, oscale);
-            }
-        } else {
-            float2 r = make_float2(0.0f, 0.0f);
-            if (n <= half) {
-                const float2 c = chirp[n];
-                float2 X = cmul(v, c);
-                const float w = n == 0 ? w0 : w1;
-                X = make_float2(X.x * w, X.y * w);
-                if constexpr (FILTER) X = cmul(X, F[n]);
-                r = cmul(X, make_float2(c.x, -c.y));
-            }
-            brow[n] = r;
-        }
+        if (n < S) store_fmt(out, orow + n, u[n1], ofmt, oscale);
     }
 }
 
@@ -775,42 +772,18 @@ static bool launch_long_n1(const HilbertInput& in, int rows, int S, const LongHi
     const bool inplace = hilbert_long_in_place(S, ofmt);
     const int chunk = inplace ? rows : (int)ws_rows;
     if (!inplace && (!ws || chunk <= 0)) CUT_FAIL("Hilbert: no workspace for rows of " << S << " samples");
+    if (!pow2 && !t.H) CUT_FAIL("Hilbert: no convolution table for rows of " << S << " samples");
     for (int r0 = 0; r0 < rows; r0 += chunk) {
         const int nr = std::min(chunk, rows - r0);
         float2* buf = inplace ? static_cast<float2*>(out) + (size_t)r0 * S : ws;
         const dim3 gp(LN2 / 256, nr), gr(N1, nr);
-        auto fwd = [&](int pre) {
-            if (pre == LPRE_BUF) {
-                long_fwd_pass_kernel<N1, 0, LPRE_BUF><<<gp, 256, 0, st>>>(d, r0, S, t.twN, t.chirp, buf);
-                return;
-            }
-#define CUT_LF(K, P) long_fwd_pass_kernel<N1, K, P><<<gp, 256, 0, st>>>(d, r0, S, t.twN, t.chirp, buf)
-            if (pre == LPRE_REAL) {
-                if (in.kind == 0) CUT_LF(0, LPRE_REAL); else if (in.kind == 1) CUT_LF(1, LPRE_REAL); else CUT_LF(2, LPRE_REAL);
-            } else {
-                if (in.kind == 0) CUT_LF(0, LPRE_REAL_CHIRP); else if (in.kind == 1) CUT_LF(1, LPRE_REAL_CHIRP);
-                else CUT_LF(2, LPRE_REAL_CHIRP);
-            }
-#undef CUT_LF
-        };
-        if (pow2) {
-            fwd(LPRE_REAL);
-            if (F) long_row_kernel<LROW_HILBERT, true><<<gr, 256, 0, st>>>(buf, N1, S, t.tw2, nullptr, F);
-            else long_row_kernel<LROW_HILBERT, false><<<gr, 256, 0, st>>>(buf, N1, S, t.tw2, nullptr, nullptr);
-            long_inv_pass_kernel<N1, LPOST_OUT, false><<<gp, 256, 0, st>>>(r0, S, t.twN, t.chirp, nullptr, buf, out,
-                                                                          ofmt, oscale);
-        } else {
-            fwd(LPRE_REAL_CHIRP);
-            long_row_kernel<LROW_MUL, false><<<gr, 256, 0, st>>>(buf, N1, S, t.tw2, t.V, nullptr);
-            if (F) long_inv_pass_kernel<N1, LPOST_MID, true><<<gp, 256, 0, st>>>(r0, S, t.twN, t.chirp, F, buf, out,
-                                                                                ofmt, oscale);
-            else long_inv_pass_kernel<N1, LPOST_MID, false><<<gp, 256, 0, st>>>(r0, S, t.twN, t.chirp, nullptr, buf,
-                                                                               out, ofmt, oscale);
-            fwd(LPRE_BUF);
-            long_row_kernel<LROW_MUL, false><<<gr, 256, 0, st>>>(buf, N1, S, t.tw2, t.Wt, nullptr);
-            long_inv_pass_kernel<N1, LPOST_OUT_CHIRP, false><<<gp, 256, 0, st>>>(r0, S, t.twN, t.chirp, nullptr, buf,
-                                                                                out, ofmt, oscale);
-        }
+        if (in.kind == 0) long_fwd_pass_kernel<N1, 0><<<gp, 256, 0, st>>>(d, r0, S, t.twN, buf);
+        else if (in.kind == 1) long_fwd_pass_kernel<N1, 1><<<gp, 256, 0, st>>>(d, r0, S, t.twN, buf);
+        else long_fwd_pass_kernel<N1, 2><<<gp, 256, 0, st>>>(d, r0, S, t.twN, buf);
+        if (!pow2) long_row_kernel<LROW_MUL, false><<<gr, 256, 0, st>>>(buf, N1, S, t.tw2, t.H, nullptr);
+        else if (F) long_row_kernel<LROW_HILBERT, true><<<gr, 256, 0, st>>>(buf, N1, S, t.tw2, nullptr, F);
+        else long_row_kernel<LROW_HILBERT, false><<<gr, 256, 0, st>>>(buf, N1, S, t.tw2, nullptr, nullptr);
+        long_inv_pass_kernel<N1><<<gp, 256, 0, st>>>(r0, S, t.twN, buf, out, ofmt, oscale);
         CUT_HIP_CHECK(hipGetLastError());
     }
     return true;
@@ -822,6 +795,7 @@ bool launch_hilbert_long(const HilbertInput& in, int rows, int S, const LongHilb
     if (rows <= 0) return true;
     if (ofmt != IQ_F32 && ofmt != IQ_F16 && ofmt != IQ_BF16) CUT_FAIL("Hilbert: unknown output format " << ofmt);
     if (!hilbert_long(S) || !hilbert_supported(S)) CUT_FAIL("Hilbert: sample count " << S << " not supported");
+    if (!t.twN || !t.tw2) CUT_FAIL("Hilbert: twiddle tables of " << S << " samples missing");
     switch (hilbert_long_n(S) / LN2) {
         case 2: return launch_long_n1<2>(in, rows, S, t, d_F, out, ofmt, oscale, ws, ws_rows, st);
         case 4: return launch_long_n1<4>(in, rows, S, t, d_F, out, ofmt, oscale, ws, ws_rows, st);

@@ -7,6 +7,7 @@
 // sizes, channel-map entries and filter length are validated before any
 // kernel runs, and nothing synchronises the device inside the pipeline.
 #include "pipeline.hpp"
+#include "tuning.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -154,25 +155,19 @@ bool RfPipeline::init(uint raw_len, uint raw_channels, uint S, uint C, uint T, i
         CUT_HIP_CHECK(hipMemcpy(rowsum_.ptr, rs.data(), rs.size() * 4, hipMemcpyHostToDevice));
         scale_ = 1.0f / (float)T;
     }
-    // FFT plan: twiddles for power-of-two lengths, chirp-z tables for the
-    // others; rows longer than 4096 also take the 4096-point twiddles of their
-    // row steps (unsupported lengths fail at hilbert())
-    chirp_.release(); bV_.release(); bW_.release(); tw2_.release(); long_ws_.release();
-    if (hilbert_long((int)S) && hilbert_supported((int)S)) {
-        if (!tw2_.ensure((size_t)4096 * sizeof(float2)) || !launch_twiddles(tw2_.as<float2>(), 4096, stream()))
+    // FFT plan: twiddles of the transform length (S for powers of two, the
+    // convolution length M >= 2S - 1 otherwise, whose kernel spectrum is
+    // built below and with every filter change); rows longer than 4096 also
+    // take the 4096-point twiddles of their row steps (unsupported lengths
+    // fail at hilbert())
+    conv_H_.release(); tw2_.release(); long_ws_.release();
+    if (hilbert_supported((int)S)) {
+        if (hilbert_long((int)S) &&
+            (!tw2_.ensure((size_t)4096 * sizeof(float2)) || !launch_twiddles(tw2_.as<float2>(), 4096, stream())))
             return false;
-    }
-    if (hilbert_pow2((int)S) || (hilbert_long((int)S) && hilbert_supported((int)S) && (S & (S - 1)) == 0)) {
-        if (!tw_.ensure((size_t)S * sizeof(float2))) return false;
-        if (!launch_twiddles(tw_.as<float2>(), (int)S, stream())) return false;
-    } else if (hilbert_supported((int)S)) {
-        const size_t M = (size_t)1 << bluestein_log2((int)S);
-        if (!tw_.ensure(M * sizeof(float2)) || !chirp_.ensure((size_t)S * sizeof(float2)) ||
-            !bV_.ensure(M * sizeof(float2)) || !bW_.ensure(M * sizeof(float2)))
-            return false;
-        if (!launch_bluestein_tables((int)S, chirp_.as<float2>(), bV_.as<float2>(), bW_.as<float2>(),
-                                     tw_.as<float2>(), stream()))
-            return false;
+        const size_t N = (size_t)hilbert_fft_len((int)S);
+        if (!tw_.ensure(N * sizeof(float2)) || !launch_twiddles(tw_.as<float2>(), (int)N, stream())) return false;
+        if (hilbert_conv((int)S) && !conv_H_.ensure(N * sizeof(float2))) return false;
     } else {
         tw_.release();
     }
@@ -185,6 +180,19 @@ bool RfPipeline::init(uint raw_len, uint raw_channels, uint S, uint C, uint T, i
         filter_on_ = false;
         if (h.size() <= S) return set_match_filter(h.data(), h.size());
     }
+    return build_conv_table();
+}
+
+// Kernel spectrum of the convolution form (lengths that are not powers of
+// two) for the current filter.
+bool RfPipeline::build_conv_table()
+{
+    if (!conv_H_.ptr) return true;
+    if (!conv_tmp_.ensure(std::max(conv_tmp_.bytes, conv_tables_tmp_bytes((int)S_)))) return false;
+    const int L = filter_on_ ? (int)filter_host_.size() : 0;
+    if (!launch_conv_tables(filter_taps_.as<float>(), L, (int)S_, conv_tmp_.ptr, conv_H_.as<float2>(), stream()))
+        return false;
+    CUT_HIP_CHECK(hipStreamSynchronize(stream()));
     return true;
 }
 
@@ -196,7 +204,7 @@ bool RfPipeline::deinit()
     }
     init_ = false;
     h8_.release(); hf_.release(); rowsum_.release(); tw_.release();
-    chirp_.release(); bV_.release(); bW_.release(); tw2_.release(); long_ws_.release();
+    conv_H_.release(); conv_tmp_.release(); tw2_.release(); long_ws_.release();
     filter_spec_.release(); filter_taps_.release();
     filter_host_.clear();
     filter_on_ = false;
@@ -241,9 +249,10 @@ bool RfPipeline::set_match_filter(const float* h, size_t length)
 {
     if (!init_) CUT_FAIL("Session not initialized.");
     if (length == 0 || !h) {
+        const bool was_on = filter_on_;
         filter_on_ = false;
         filter_host_.clear();
-        return true;
+        return was_on ? build_conv_table() : true;
     }
     if (length > S_) CUT_FAIL("Failed to load match filter: length " << length << " exceeds " << S_ << " samples.");
     if (filter_on_ && filter_host_.size() == length && std::equal(filter_host_.begin(), filter_host_.end(), h) &&
@@ -256,7 +265,7 @@ bool RfPipeline::set_match_filter(const float* h, size_t length)
         return false;
     CUT_HIP_CHECK(hipStreamSynchronize(stream()));
     filter_on_ = true;
-    return true;
+    return build_conv_table();
 }
 
 bool RfPipeline::convert_decode(const void* d_raw, int type, float* d_out, int out_stride, bool decode, hipStream_t st)
@@ -280,7 +289,6 @@ bool RfPipeline::hilbert(const HilbertInput& in, void* d_out, int fmt, float sca
     if (!init_) CUT_FAIL("Session not initialized.");
     if (!hilbert_supported((int)S_)) CUT_FAIL("Hilbert: sample count " << S_ << " is not supported by the FFT.");
     const float2* F = filter_on_ ? filter_spec_.as<float2>() : nullptr;
-    if (hilbert_pow2((int)S_)) return launch_hilbert(in, (int)(C_ * T_), (int)S_, tw_.as<float2>(), F, d_out, fmt, scale, st);
     if (hilbert_long((int)S_)) {
         const int rows = (int)(C_ * T_);
         size_t ws_rows = 0;
@@ -289,12 +297,21 @@ bool RfPipeline::hilbert(const HilbertInput& in, void* d_out, int fmt, float sca
             if (!long_ws_.ensure(std::max(long_ws_.bytes, ws_rows * hilbert_long_n((int)S_) * sizeof(float2))))
                 return false;
         }
-        const LongHilbertTables t{tw_.as<float2>(), tw2_.as<float2>(), chirp_.as<float2>(), bV_.as<float2>(),
-                                  bW_.as<float2>()};
+        const LongHilbertTables t{tw_.as<float2>(), tw2_.as<float2>(), conv_H_.as<float2>()};
         return launch_hilbert_long(in, rows, (int)S_, t, F, d_out, fmt, scale, long_ws_.as<float2>(), ws_rows, st);
     }
-    const BluesteinTables bt{chirp_.as<float2>(), bV_.as<float2>(), bW_.as<float2>(), tw_.as<float2>()};
-    return launch_hilbert_bluestein(in, (int)(C_ * T_), (int)S_, bt, F, d_out, fmt, scale, st);
+    // FFT tile (values per thread of the LDS transforms): forced by the
+    // options, else tuned per (length, rows, input, filter, format)
+    const int rows = (int)(C_ * T_);
+    const bool pow2 = !hilbert_conv((int)S_);  // powers of two 16..4096
+    auto run = [&](int vpt) {
+        return pow2 ? launch_hilbert(in, rows, (int)S_, tw_.as<float2>(), F, d_out, fmt, scale, vpt, st)
+                    : launch_hilbert_conv(in, rows, (int)S_, tw_.as<float2>(), conv_H_.as<float2>(), d_out, fmt, scale,
+                                          vpt, st);
+    };
+    if (options().fft_vpt) return run(options().fft_vpt);
+    const TuneKey key{(long long)S_, rows, in.kind, in.stride, F != nullptr, fmt, 0, 0, 0, 0, 0, 0};
+    return tuned_run(TUNE_FFT_TILE, key, 16, {16, 8}, st, true, run);
 }
 
 bool RfPipeline::convert_hilbert(const void* d_raw, int type, void* d_out, int fmt, float scale, hipStream_t st)

@@ -19,6 +19,7 @@ struct Options {
     int das_cubic = 1;      // 1 Catmull-Rom (reference), 0 linear
     int das_cf_hercules = 0;
     int iq_format = 0;      // IqFormat of the IQ handed from the Hilbert transform to the DAS
+    int fft_vpt = 0;        // FFT tile of the Hilbert kernels: 0 tuned, 8 or 16 values per thread
 };
 Options& options();
 
@@ -47,6 +48,7 @@ public:
     // signal of int16 data filtered by the current taps stays below 32768.
     float fp16_scale() const;
     bool check_map(uint raw_channels) const;
+    bool build_conv_table();
 
     uint S() const { return S_; }
     uint C() const { return C_; }
@@ -64,7 +66,7 @@ private:
     DeviceBuffer h8_, hf_, rowsum_;
     float scale_ = 1.0f;
     DeviceBuffer tw_;
-    DeviceBuffer chirp_, bV_, bW_;  // chirp-z tables (lengths that are not powers of two)
+    DeviceBuffer conv_H_, conv_tmp_;  // convolution kernel spectrum (lengths that are not powers of two)
     DeviceBuffer tw2_, long_ws_;    // rows > 4096: 4096-point twiddles, workspace of the HBM-staged steps
     DeviceBuffer filter_taps_, filter_spec_;
     std::vector<float> filter_host_;

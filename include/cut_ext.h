@@ -79,6 +79,20 @@ CUT_EXT_EXPORT bool cut_beamform_batch(const CutFrame* frames, int n, int mode, 
  * each taking a share of the voxel's outer loop) per frame shape; each is
  * timed on first use (that call synchronises) and the fastest kept. */
 CUT_EXT_EXPORT bool cut_set_autotune(int on);
+/* FFT tile of the Hilbert transforms (rows of up to 4096 samples): values
+ * per thread in the LDS passes, 8 or 16; 0 (default) = tuned per row length,
+ * row count, input kind, filter and output format on first use (with
+ * autotuning on), else 16. */
+CUT_EXT_EXPORT bool cut_set_fft_tile(int values_per_thread);
+/* The tuned launch shapes (DAS split, waves per voxel group, FFT tile) differ
+ * in the rounding of the results (~1e-7 relative), so two processes that tune
+ * independently can differ in the last bits.  cut_tuning_save writes this
+ * process's choices to a text file, cut_tuning_load reads such a file (its
+ * choices replace tuning for those shapes); with the environment variable
+ * CUT_TUNING_FILE set, the library reads that file on first use and rewrites
+ * it after every new choice, so processes sharing it compute the same bits. */
+CUT_EXT_EXPORT bool cut_tuning_save(const char* path);
+CUT_EXT_EXPORT bool cut_tuning_load(const char* path);
 /* Launch shape the last graph-mode batch used: resident workgroups per CU
  * (0 = one workgroup per tile) and work order (0 heaviest first, 1 frame by frame). */
 CUT_EXT_EXPORT bool cut_batch_launch_shape(int* grid_per_cu, int* order);

@@ -134,3 +134,27 @@ def test_c_api_call_order_errors(lib, capfd):
     assert "OGL buffers not registered" in capfd.readouterr().err
     assert api.register_cuda_buffers(list(range(1, 18)), 99) is False  # > 16 buffers
     assert "Too many RF data buffers" in capfd.readouterr().err
+
+
+def test_tuning_table_text_and_fft_tile_values(lib, capfd, tmp_path):
+    """The tuning table's text form (no GPU needed): load, save, reject
+    malformed lines; the FFT tile accepts 0, 8 and 16 only."""
+    from cuda_ultrasound_toolkit_amd import api
+    good = tmp_path / "good.txt"
+    good.write_text("2 4096 16384 1 1 1 0 0 0 0 0 0 0 8\n0 128 128 128 128 128 4096 1 0 0 0 0 0 16\n")
+    assert api.tuning_load(good)
+    out = tmp_path / "out.txt"
+    assert api.tuning_save(out)
+    lines = set(out.read_text().splitlines())
+    assert set(good.read_text().splitlines()) <= lines
+    bad = tmp_path / "bad.txt"
+    bad.write_text("7 1 2 3\n")
+    assert api.tuning_load(bad) is False
+    assert "malformed line 1" in capfd.readouterr().err
+    assert api.tuning_load(tmp_path / "missing.txt") is False
+    assert "cannot read" in capfd.readouterr().err
+    for v in (0, 8, 16):
+        assert api.set_fft_tile(v)
+    assert api.set_fft_tile(4) is False
+    assert "FFT tile must be 0" in capfd.readouterr().err
+    assert api.set_fft_tile(0)

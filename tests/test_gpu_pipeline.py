@@ -123,9 +123,12 @@ def test_convert_f32_and_i16(torch_dev, orc):
                                     # lengths that are not powers of two: chirp-z path
                                     (4095, 0), (3000, 64), (2049, 5), (1234, 17), (1000, 0), (100, 7),
                                     (17, 3), (12, 0), (3, 0), (1, 0)])
-def test_hilbert_matches_oracle(torch_dev, orc, S, taps):
+@pytest.mark.parametrize("vpt", [16, 8])
+def test_hilbert_matches_oracle(torch_dev, orc, S, taps, vpt):
+    """Both FFT tiles (8 or 16 values per thread) against the oracle."""
     import torch
-    print(f"hilbert S={S} taps={taps}", flush=True)
+    print(f"hilbert S={S} taps={taps} fft tile {vpt}", flush=True)
+    assert api.set_fft_tile(vpt)
     T, Cc = 4, 3
     acq = configs.linear_2d(T=T, S=S, generate=False)
     acq.bp.dec_data_dim[1] = Cc
@@ -147,6 +150,7 @@ def test_hilbert_matches_oracle(torch_dev, orc, S, taps):
     out2 = torch.zeros_like(out)
     assert api.rf_hilbert(_t(xc.view(np.float32), torch_dev), 2, out2)
     torch.cuda.synchronize()
+    assert api.set_fft_tile(0)
     assert np.array_equal(out2.cpu().numpy(), out.cpu().numpy())
     assert api.rf_set_match_filter(None)
 
