@@ -120,15 +120,17 @@ bool hilbert_supported(int S);  // 1 <= S <= 4096, powers of two up to 65536, ot
 bool hilbert_pow2(int S);       // powers of two 16..4096 take the radix-16 kernel
 // Other lengths: the operator as a cyclic convolution, evaluated with
 // M = 2^conv_log2(S) >= 2S - 1 point FFTs of the zero-padded rows; H = the
-// M-point spectrum of the operator's kernel (with weights, filter and 1/M),
+// M-point spectrum of the operator's kernel h (with weights, filter and 1/M),
 // built in FP64 by launch_conv_tables (d_tmp: conv_tables_tmp_bytes(S)); d_h,
-// L: the match filter taps (L = 0: none).
+// L: the match filter taps (L = 0: none).  pair: the table of the LDS row-pair
+// kernel, M float4 (H_r, H_i) = the spectra of Re h and Im h; else M float2
+// H (the long path).
 int conv_log2(int S);
 bool hilbert_conv(int S);     // the length takes the convolution form
 int hilbert_fft_len(int S);   // transform length: S, or 2^conv_log2(S) for the convolution form
 size_t conv_tables_tmp_bytes(int S);
-bool launch_conv_tables(const float* d_h, int L, int S, void* d_tmp, float2* d_H, hipStream_t st);
-bool launch_hilbert_conv(const HilbertInput& in, int rows, int S, const float2* d_tw, const float2* d_H, void* out,
+bool launch_conv_tables(const float* d_h, int L, int S, void* d_tmp, void* d_H, bool pair, hipStream_t st);
+bool launch_hilbert_conv(const HilbertInput& in, int rows, int S, const float2* d_tw, const float4* d_H, void* out,
                          int ofmt, float oscale, int vpt, hipStream_t st);
 // Rows of more than 4096 samples: N-point transforms (N = S for powers of
 // two, else the convolution length 2^conv_log2(S)) as N / 4096 x 4096 steps

@@ -359,54 +359,98 @@ __global__ __launch_bounds__(fft_block(LOGS, VPT)) void hilbert_pair_kernel(Hilb
 // length S with g = IDFT_S(G): y[n] = sum_m x[m] g[(n - m) mod S].  With the row
 // zero-padded to M = 2^conv_log2(S) >= 2S - 1 and h the kernel laid out over
 // the offsets -(S-1)..S-1 (h[j mod M] = g[j mod S]), the cyclic convolution of
-// length M equals it for n < S (no offset wraps around), so
-//   y[n] = IFFT_M(FFT_M(x) . H)[n],  H = FFT_M(h) / M,
-// two M-point transforms per row held in LDS, with H computed once per length
-// and filter in FP64 (conv_tables).
+// length M equals it for n < S (no offset wraps around):
+//   y[n] = IFFT_M(FFT_M(x) . H)[n],  H = FFT_M(h) / M.
+// Rows are taken in pairs a, b.  With h = h_r + i h_i (both real) and
+// H_r = FFT_M(h_r) / M, H_i = FFT_M(h_i) / M, one forward transform of
+// x = x_a + i x_b and two inverse ones give both rows:
+//   P = IFFT_M(X . H_r) = conv(x_a, h_r) + i conv(x_b, h_r),
+//   Q = IFFT_M(X . H_i) = conv(x_a, h_i) + i conv(x_b, h_i),
+//   y_a = Re P + i Re Q,  y_b = Im P + i Im Q
+// (three transforms per pair instead of four).  HP[k] = (H_r[k], H_i[k]) is
+// built once per length and filter in FP64 (conv_tables).  The first pass of
+// P's inverse reads X[k] from LDS and keeps X[k] H_i[k] in registers: the
+// first pass of Q's inverse loads the same indices in the same order, as the
+// last passes of the two inverses store the same indices, so P's outputs stay
+// in registers until Q's are formed and each output element is written once.
+// A lone last row (odd row count) pairs with zeros; rounding of the shared
+// transforms scales with the larger row of the pair (FP32, ~1e-7).
 template <int LOGM, int KIND, int VPT>
 __global__ __launch_bounds__(fft_block(LOGM, VPT)) void hilbert_conv_kernel(HilbertDev in, int rows, int S,
                                                                            const float2* __restrict__ tw,
-                                                                           const float2* __restrict__ H,
+                                                                           const float4* __restrict__ HP,
                                                                            void* __restrict__ out, int ofmt,
                                                                            float oscale)
 {
     constexpr int M = 1 << LOGM;
     constexpr int P = M / VPT;
-    constexpr int RPW = P >= 256 ? 1 : 256 / P;
+    constexpr int RPW = P >= 256 ? 1 : 256 / P;  // row pairs per workgroup
     constexpr int PADDED = M + M / 16;
     __shared__ float2 buf[RPW * PADDED];
 
     const int tid = threadIdx.x % P;
     const int rsub = threadIdx.x / P;
-    const int row = blockIdx.x * RPW + rsub;
-    const bool active = row < rows;
+    const int pair = blockIdx.x * RPW + rsub;
+    const int ra = 2 * pair, rb = 2 * pair + 1;
+    const bool act_a = ra < rows, act_b = rb < rows;
+    const int rra = act_a ? ra : 0, rrb = act_b ? rb : 0;
     float2* lds = buf + rsub * PADDED;
-    const int rrow = active ? row : 0;
-    const void* base = row_base<KIND>(in, rrow, S);
+    const void* base_a = row_base<KIND>(in, rra, S);
+    const void* base_b = row_base<KIND>(in, rrb, S);
     const unsigned istride = (unsigned)in.stride;
 
     auto load_x = [&](int n) {
-        return make_float2(active && n < S ? load_at<KIND>(base, istride, (unsigned)n) : 0.0f, 0.0f);
+        const bool in_row = n < S;
+        return make_float2(act_a && in_row ? load_at<KIND>(base_a, istride, (unsigned)n) : 0.0f,
+                           act_b && in_row ? load_at<KIND>(base_b, istride, (unsigned)n) : 0.0f);
     };
-    auto load_lds = [&](int i) { return lds[lds_idx(i)]; };
-    auto store_H = [&](int k, float2 v) { lds[lds_idx(k)] = cmul(v, H[k]); };
-    const size_t obase = (size_t)rrow * S;
-    auto store_out = [&](int n, float2 v) {
-        if (!active || n >= S) return;
+    auto store_lds = [&](int i, float2 v) { lds[lds_idx(i)] = v; };
+    fft_passes<LOGM, false, VPT>(tid, tw, lds, load_x, false, store_lds, false);
+
+    // P: X H_r from LDS, X H_i kept for Q (each pass's loads are fully unrolled,
+    // so the counters below index registers)
+    float2 xq[VPT];
+    int nq = 0;
+    auto load_p = [&](int k) {
+        const float2 x = lds[lds_idx(k)];
+        const float4 h = HP[k];
+        xq[nq++] = cmul(x, make_float2(h.z, h.w));
+        return cmul(x, make_float2(h.x, h.y));
+    };
+    float2 pv[VPT];
+    int np = 0;
+    auto store_p = [&](int n, float2 v) { pv[np++] = v; };
+    const float2* tw_p = tw;
+    __asm__ volatile("" : "+s"(tw_p));  // twiddles reloaded per transform (see hilbert_pair_kernel)
+    fft_passes<LOGM, true, VPT>(tid, tw_p, lds, load_p, true, store_p, true);
+
+    // Q from the kept products; its last pass writes both rows
+    int lq = 0;
+    auto load_q = [&](int) { return xq[lq++]; };
+    const size_t esz = ofmt == IQ_F32 ? sizeof(float2) : sizeof(uint32_t);
+    char* const out_a = static_cast<char*>(out) + (size_t)rra * S * esz;
+    char* const out_b = static_cast<char*>(out) + (size_t)rrb * S * esz;
+    auto put = [&](char* orow, int n, float2 v) {
         if (ofmt == IQ_F32) {
-            static_cast<float2*>(out)[obase + n] = v;
+            reinterpret_cast<float2*>(orow)[(unsigned)n] = v;
         } else if (ofmt == IQ_F16) {
-            static_cast<__half2*>(out)[obase + n] = __floats2half2_rn(v.x * oscale, v.y * oscale);
+            reinterpret_cast<__half2*>(orow)[(unsigned)n] = __floats2half2_rn(v.x * oscale, v.y * oscale);
         } else {
-            static_cast<uint32_t*>(out)[obase + n] = bf16_bits(v.x) | (bf16_bits(v.y) << 16);
+            reinterpret_cast<uint32_t*>(orow)[(unsigned)n] = bf16_bits(v.x) | (bf16_bits(v.y) << 16);
         }
     };
-    fft_passes<LOGM, false, VPT>(tid, tw, lds, load_x, false, store_H, false);
-    // the inverse reads the twiddles through its own opaque copy of the
-    // pointer (see hilbert_pair_kernel)
-    const float2* tw2 = tw;
-    __asm__ volatile("" : "+s"(tw2));
-    fft_passes<LOGM, true, VPT>(tid, tw2, lds, load_lds, true, store_out, true);
+    int sq = 0;
+    auto store_q = [&](int n, float2 q) {
+        const float2 p = pv[sq++];
+        if (n >= S) return;
+        if (act_a) put(out_a, n, make_float2(p.x, q.x));
+        if (act_b) put(out_b, n, make_float2(p.y, q.y));
+    };
+    const float2* tw_q = tw;
+    __asm__ volatile("" : "+s"(tw_q));
+    // first_lds: the barrier inside Q's first pass orders its LDS stores after
+    // the loads of P's last pass
+    fft_passes<LOGM, true, VPT>(tid, tw_q, lds, load_q, true, store_q, true);
 }
 
 // Kernel spectrum of the convolution form, in FP64 (exact integer angle
@@ -463,6 +507,27 @@ __global__ void conv_H_kernel(const double2* __restrict__ g, int S, int M, float
     H[k] = make_float2((float)(re / M), (float)(im / M));
 }
 
+// The same spectrum split by the real and imaginary parts of the kernel, for
+// the row-pair kernel: HP[k] = (H_r[k], H_i[k]), H_r = FFT_M(Re h) / M,
+// H_i = FFT_M(Im h) / M (so H = H_r + i H_i).
+__global__ void conv_HP_kernel(const double2* __restrict__ g, int S, int M, float4* __restrict__ HP)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= M) return;
+    double rr = 0.0, ri = 0.0, ir = 0.0, ii = 0.0;
+    for (int j = -(S - 1); j < S; ++j) {
+        const double2 v = g[j < 0 ? j + S : j];
+        const unsigned long long e = ((unsigned long long)(j < 0 ? j + M : j) * (unsigned long long)k) & (M - 1);
+        double sn, cs;
+        sincospi(2.0 * (double)e / (double)M, &sn, &cs);
+        rr += v.x * cs;
+        ri -= v.x * sn;
+        ir += v.y * cs;
+        ii -= v.y * sn;
+    }
+    HP[k] = make_float4((float)(rr / M), (float)(ri / M), (float)(ir / M), (float)(ii / M));
+}
+
 int conv_log2(int S)
 {
     int lg = 4;
@@ -472,14 +537,15 @@ int conv_log2(int S)
 
 size_t conv_tables_tmp_bytes(int S) { return (size_t)(S / 2 + 1 + S) * sizeof(double2); }
 
-bool launch_conv_tables(const float* d_h, int L, int S, void* d_tmp, float2* d_H, hipStream_t st)
+bool launch_conv_tables(const float* d_h, int L, int S, void* d_tmp, void* d_H, bool pair, hipStream_t st)
 {
     const int M = 1 << conv_log2(S);
     double2* G = static_cast<double2*>(d_tmp);
     double2* g = G + (S / 2 + 1);
     conv_G_kernel<<<(S / 2 + 1 + 127) / 128, 128, 0, st>>>(d_h, L, S, G);
     conv_g_kernel<<<(S + 127) / 128, 128, 0, st>>>(G, S, g);
-    conv_H_kernel<<<(M + 127) / 128, 128, 0, st>>>(g, S, M, d_H);
+    if (pair) conv_HP_kernel<<<(M + 127) / 128, 128, 0, st>>>(g, S, M, static_cast<float4*>(d_H));
+    else conv_H_kernel<<<(M + 127) / 128, 128, 0, st>>>(g, S, M, static_cast<float2*>(d_H));
     CUT_HIP_CHECK(hipGetLastError());
     return true;
 }
@@ -539,14 +605,15 @@ int hilbert_fft_len(int S) { return hilbert_conv(S) ? 1 << conv_log2(S) : S; }
 bool hilbert_supported(int S) { return S >= 1 && (S <= 4096 || hilbert_long_n(S) <= HILBERT_LONG_MAX_N); }
 
 template <int LOGM, int VPT>
-static bool launch_conv_m(const HilbertInput& in, int rows, int S, const float2* tw, const float2* H, void* out,
+static bool launch_conv_m(const HilbertInput& in, int rows, int S, const float2* tw, const float4* H, void* out,
                           int ofmt, float oscale, hipStream_t st)
 {
     constexpr int M = 1 << LOGM;
     constexpr int P = M / VPT;
-    constexpr int RPW = P >= 256 ? 1 : 256 / P;
+    constexpr int RPW = P >= 256 ? 1 : 256 / P;  // row pairs per workgroup
     HilbertDev d{in.ptr, in.stride, in.map, in.R0, in.C};
-    dim3 grid((rows + RPW - 1) / RPW);
+    const int pairs = (rows + 1) / 2;
+    dim3 grid((pairs + RPW - 1) / RPW);
     dim3 block(P * RPW);
 #define CUT_CONV(K) hilbert_conv_kernel<LOGM, K, VPT><<<grid, block, 0, st>>>(d, rows, S, tw, H, out, ofmt, oscale)
     if (in.kind == 0) CUT_CONV(0); else if (in.kind == 1) CUT_CONV(1); else CUT_CONV(2);
@@ -556,7 +623,7 @@ static bool launch_conv_m(const HilbertInput& in, int rows, int S, const float2*
 }
 
 template <int VPT>
-static bool launch_conv_vpt(const HilbertInput& in, int rows, int S, const float2* tw, const float2* H, void* out,
+static bool launch_conv_vpt(const HilbertInput& in, int rows, int S, const float2* tw, const float4* H, void* out,
                             int ofmt, float oscale, hipStream_t st)
 {
     switch (conv_log2(S)) {
@@ -574,7 +641,7 @@ static bool launch_conv_vpt(const HilbertInput& in, int rows, int S, const float
     }
 }
 
-bool launch_hilbert_conv(const HilbertInput& in, int rows, int S, const float2* d_tw, const float2* d_H, void* out,
+bool launch_hilbert_conv(const HilbertInput& in, int rows, int S, const float2* d_tw, const float4* d_H, void* out,
                          int ofmt, float oscale, int vpt, hipStream_t st)
 {
     if (rows <= 0) return true;

@@ -167,7 +167,8 @@ bool RfPipeline::init(uint raw_len, uint raw_channels, uint S, uint C, uint T, i
             return false;
         const size_t N = (size_t)hilbert_fft_len((int)S);
         if (!tw_.ensure(N * sizeof(float2)) || !launch_twiddles(tw_.as<float2>(), (int)N, stream())) return false;
-        if (hilbert_conv((int)S) && !conv_H_.ensure(N * sizeof(float2))) return false;
+        if (hilbert_conv((int)S) && !conv_H_.ensure(N * (hilbert_long((int)S) ? sizeof(float2) : sizeof(float4))))
+            return false;
     } else {
         tw_.release();
     }
@@ -190,7 +191,8 @@ bool RfPipeline::build_conv_table()
     if (!conv_H_.ptr) return true;
     if (!conv_tmp_.ensure(std::max(conv_tmp_.bytes, conv_tables_tmp_bytes((int)S_)))) return false;
     const int L = filter_on_ ? (int)filter_host_.size() : 0;
-    if (!launch_conv_tables(filter_taps_.as<float>(), L, (int)S_, conv_tmp_.ptr, conv_H_.as<float2>(), stream()))
+    if (!launch_conv_tables(filter_taps_.as<float>(), L, (int)S_, conv_tmp_.ptr, conv_H_.ptr, !hilbert_long((int)S_),
+                            stream()))
         return false;
     CUT_HIP_CHECK(hipStreamSynchronize(stream()));
     return true;
@@ -306,7 +308,7 @@ bool RfPipeline::hilbert(const HilbertInput& in, void* d_out, int fmt, float sca
     const bool pow2 = !hilbert_conv((int)S_);  // powers of two 16..4096
     auto run = [&](int vpt) {
         return pow2 ? launch_hilbert(in, rows, (int)S_, tw_.as<float2>(), F, d_out, fmt, scale, vpt, st)
-                    : launch_hilbert_conv(in, rows, (int)S_, tw_.as<float2>(), conv_H_.as<float2>(), d_out, fmt, scale,
+                    : launch_hilbert_conv(in, rows, (int)S_, tw_.as<float2>(), conv_H_.as<float4>(), d_out, fmt, scale,
                                           vpt, st);
     };
     if (options().fft_vpt) return run(options().fft_vpt);
