@@ -313,12 +313,15 @@ __global__ __launch_bounds__(256) void decode_i8_kernel(const int16_t* __restric
 }
 
 // Streaming form of decode_i8_kernel for T <= 128 (one k chunk) and 16-byte
-// aligned rows: a fixed grid of workgroups walks the (channel, 64-sample)
+// aligned rows: a fixed grid of workgroups walks the (channel, NT-sample)
 // tiles in order (tile = blockIdx.x + i * gridDim.x), keeps H in registers,
 // and issues the next tile's int16 loads before the current tile's MFMA and
 // stores, so HBM reads stay in flight while a workgroup computes and writes.
-// Results are identical to decode_i8_kernel (same products, same integer sums).
-template <int OUT_STRIDE>
+// Staging: thread (kq, sg) loads samples sg*8 .. sg*8+7 of rows kq*RPT ..
+// kq*RPT+RPT-1 (16-byte loads; NT/8 threads cover a row segment of NT
+// samples).  Results are identical to decode_i8_kernel (same products, same
+// integer sums).
+template <int OUT_STRIDE, int NT>
 __global__ __launch_bounds__(256, 3) void decode_i8_stream_kernel(const int16_t* __restrict__ raw,
                                                                const int16_t* __restrict__ map, size_t R0, int T,
                                                                int C, int S, int Kp, int nsx,
@@ -326,13 +329,20 @@ __global__ __launch_bounds__(256, 3) void decode_i8_stream_kernel(const int16_t*
                                                                const int* __restrict__ rowsum, float scale,
                                                                float* __restrict__ out)
 {
-    __shared__ __attribute__((aligned(16))) int8_t bhi[DEC_NT * DEC_ROWB];
-    __shared__ __attribute__((aligned(16))) int8_t blo[DEC_NT * DEC_ROWB];
+    constexpr int SG = NT / 8, KQ = 256 / SG, RPT = DEC_KC / KQ;
+    static_assert(RPT % 4 == 0 && RPT * KQ == DEC_KC, "staging covers the k chunk in groups of 4 rows");
+    // output tile [transmit][sample] (OTS floats per transmit row, padded) when
+    // the output is contiguous (OUT_STRIDE 1): written from the MFMA layout,
+    // then stored as float4 per lane, whole row segments per instruction
+    constexpr int OTS = NT + 4;
+    __shared__ __attribute__((aligned(16))) int8_t bhi[NT * DEC_ROWB];
+    __shared__ __attribute__((aligned(16))) int8_t blo[NT * DEC_ROWB];
+    __shared__ __attribute__((aligned(16))) float otile[OUT_STRIDE == 1 ? DEC_KC * OTS : 4];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int mbase = wave * 32;
     const bool mvalid = mbase < Kp;
     const int ntiles = nsx * C;
-    const int kq = tid >> 3, sg = tid & 7;
+    const int kq = tid / SG, sg = tid % SG;
 
     // A operand (H rows mbase.., 32 k per fragment) for the whole K
     v4i afr[DEC_KC / 32];
@@ -343,46 +353,49 @@ __global__ __launch_bounds__(256, 3) void decode_i8_stream_kernel(const int16_t*
             afr[kk] = *(const v4i*)(h8 + (size_t)(mbase + (lane & 31)) * Kp + kk * 32 + (lane >> 5) * 16);
     }
 
-    // rows kq*4 + j, samples sg*8 .. sg*8+7 of a tile, as packed int16 pairs
-    v4i cur[4];
+    // rows kq*RPT + j, samples sg*8 .. sg*8+7 of a tile, as packed int16 pairs
+    v4i cur[RPT];
     auto load_tile = [&](int tile) {
         const int c = tile / nsx, sx = tile - c * nsx;
         const int16_t* src = raw + (size_t)(uint16_t)map[c] * R0;
-        const int s = sx * DEC_NT + sg * 8;
+        const int s = sx * NT + sg * 8;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int k = kq * 4 + j;
+        for (int j = 0; j < RPT; ++j) {
+            const int k = kq * RPT + j;
             cur[j] = (k < T && s < S) ? *(const v4i*)(src + (size_t)k * S + s) : v4i{0, 0, 0, 0};
         }
     };
     int tile = blockIdx.x;
     if (tile < ntiles) load_tile(tile);
     for (; tile < ntiles; tile += gridDim.x) {
-        const int c = tile / nsx, s0 = (tile - c * nsx) * DEC_NT;
+        const int c = tile / nsx, s0 = (tile - c * nsx) * NT;
         __syncthreads();  // the previous tile's MFMA reads of LDS are done
-        if (kq * 4 < Kp) {
+        if (kq * RPT < Kp) {
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
-                unsigned hi = 0, lo = 0;
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const unsigned x = ((unsigned)cur[j][e >> 1] >> (16 * (e & 1))) & 0xffffu;
-                    hi |= ((x >> 8) & 0xffu) << (8 * j);
-                    lo |= ((x & 0xffu) ^ 0x80u) << (8 * j);
+                for (int j0 = 0; j0 < RPT; j0 += 4) {
+                    unsigned hi = 0, lo = 0;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const unsigned x = ((unsigned)cur[j0 + j][e >> 1] >> (16 * (e & 1))) & 0xffffu;
+                        hi |= ((x >> 8) & 0xffu) << (8 * j);
+                        lo |= ((x & 0xffu) ^ 0x80u) << (8 * j);
+                    }
+                    const int off = (sg * 8 + e) * DEC_ROWB + kq * RPT + j0;
+                    *(unsigned*)&bhi[off] = hi;
+                    *(unsigned*)&blo[off] = lo;
                 }
-                const int off = (sg * 8 + e) * DEC_ROWB + kq * 4;
-                *(unsigned*)&bhi[off] = hi;
-                *(unsigned*)&blo[off] = lo;
             }
         }
         const int next = tile + gridDim.x;
         if (next < ntiles) load_tile(next);
         __syncthreads();
-        if (!mvalid) continue;
         // one accumulator per column block: H.hi, times 256, then + H.lo
         // (|H.hi| <= 128 T, so 256 H.hi + H.lo stays far inside int32)
 #pragma unroll
-        for (int nt = 0; nt < 2; ++nt) {
+        for (int nt = 0; nt < NT / 32; ++nt) {
+            if (!mvalid) break;
             v16i acc;
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[r] = 0;
@@ -399,14 +412,27 @@ __global__ __launch_bounds__(256, 3) void decode_i8_stream_kernel(const int16_t*
                 if (kk * 32 >= Kp) break;
                 acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(afr[kk], *(const v4i*)&blo[col + kk * 32], acc, 0, 0, 0);
             }
-            const int s = s0 + nt * 32 + (lane & 31);
+            const int sl = nt * 32 + (lane & 31), s = s0 + sl;
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 const int m = mbase + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-                if (m < T && s < S) {
-                    const int total = acc[r] + 128 * rowsum[m];
-                    out[(((size_t)m * C + c) * S + s) * OUT_STRIDE] = (float)total * scale;
+                const float v = (float)(acc[r] + 128 * rowsum[m < T ? m : 0]) * scale;
+                if constexpr (OUT_STRIDE == 1) {
+                    otile[m * OTS + sl] = v;
+                } else if (m < T && s < S) {
+                    out[(((size_t)m * C + c) * S + s) * OUT_STRIDE] = v;
                 }
+            }
+        }
+        if constexpr (OUT_STRIDE == 1) {
+            // S is a multiple of 8 here, so a float4 starting below S lies inside it
+            __syncthreads();
+            constexpr int Q4 = NT / 4;
+            for (int q = tid; q < T * Q4; q += 256) {
+                const int m = q / Q4, s = s0 + (q - m * Q4) * 4;
+                if (s < S)
+                    *reinterpret_cast<float4*>(out + ((size_t)m * C + c) * S + s) =
+                        *reinterpret_cast<const float4*>(&otile[m * OTS + (s - s0)]);
             }
         }
     }
@@ -493,7 +519,7 @@ bool launch_decode(const void* raw, int type, const int16_t* d_map, size_t R0, i
     const int Kp = (T + 31) / 32 * 32;
     dim3 grid((S + DEC_NT - 1) / DEC_NT, C, (Kp + 127) / 128);
     const bool vec = ((uintptr_t)raw % 16 == 0) && (R0 % 8 == 0) && (S % 8 == 0);
-    if (type == TYPE_I16 && mode == 1 && vec && Kp <= DEC_KC) {
+    if (type == TYPE_I16 && mode == 1 && vec && Kp <= DEC_KC && (uintptr_t)out % 16 == 0) {
         // streaming form: a fixed grid of workgroups per CU walks the tiles
         static int cus = 0;
         if (cus == 0) {
@@ -501,14 +527,18 @@ bool launch_decode(const void* raw, int type, const int16_t* d_map, size_t R0, i
             CUT_HIP_CHECK(hipGetDevice(&dev));
             CUT_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
         }
-        const int nsx = (S + DEC_NT - 1) / DEC_NT;
+#ifndef CUT_DEC_SNT
+#define CUT_DEC_SNT 64
+#endif
+        constexpr int SNT = CUT_DEC_SNT;
+        const int nsx = (S + SNT - 1) / SNT;
         const int ntiles = nsx * C;
         const int wgs = std::min(ntiles, cus * DEC_STREAM_WG_PER_CU);
         const int16_t* r = (const int16_t*)raw;
         if (out_stride == 1)
-            decode_i8_stream_kernel<1><<<wgs, 256, 0, st>>>(r, d_map, R0, T, C, S, Kp, nsx, d_h8, d_rowsum, scale, out);
+            decode_i8_stream_kernel<1, SNT><<<wgs, 256, 0, st>>>(r, d_map, R0, T, C, S, Kp, nsx, d_h8, d_rowsum, scale, out);
         else
-            decode_i8_stream_kernel<2><<<wgs, 256, 0, st>>>(r, d_map, R0, T, C, S, Kp, nsx, d_h8, d_rowsum, scale, out);
+            decode_i8_stream_kernel<2, SNT><<<wgs, 256, 0, st>>>(r, d_map, R0, T, C, S, Kp, nsx, d_h8, d_rowsum, scale, out);
     } else if (type == TYPE_I16 && mode == 1) {
         const int16_t* r = (const int16_t*)raw;
 #define CUT_DEC_I8(V, O) decode_i8_kernel<V, O><<<grid, 256, 0, st>>>(r, d_map, R0, T, C, S, Kp, d_h8, d_rowsum, scale, out)
