@@ -60,27 +60,48 @@ __global__ void boxsum_x_kernel(const float* __restrict__ src, int nx, int ny, i
     a2[i] = q;
 }
 
-// sum of 16 consecutive entries along an axis of stride `step` (elements)
-__global__ void boxsum_axis_kernel(const double* __restrict__ a1, const double* __restrict__ a2, size_t n_out,
-                                   int inner, int n_in_axis, int n_out_axis, size_t step, double* __restrict__ b1,
-                                   double* __restrict__ b2)
+// Sums of 16 consecutive entries along an axis of stride `step` (elements):
+// output (outer, k, inner_i) = sum_{j<16} input (outer, k + j, inner_i), for
+// k < n_out_axis.  A thread forms BS_RUN consecutive outputs along the axis
+// from the BS_RUN + 15 inputs they cover, held in registers (19 loads per 4
+// outputs instead of 64; longer runs leave too few threads to hide the load
+// latency), each sum taken in the order j = 0..15 from 0.
+constexpr int BS_RUN = 4;
+__global__ __launch_bounds__(256) void boxsum_axis_kernel(const double* __restrict__ a1, const double* __restrict__ a2,
+                                                          size_t n_runs, int inner, int n_in_axis, int n_out_axis,
+                                                          size_t step, double* __restrict__ b1,
+                                                          double* __restrict__ b2)
 {
-    // output index = (outer, k, inner_i) with k < n_out_axis; input (outer, k + j, inner_i)
+    // run index = (outer, run, inner_i); outputs k = run * BS_RUN .. + BS_RUN - 1
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_out) return;
+    if (i >= n_runs) return;
+    const int runs_axis = (n_out_axis + BS_RUN - 1) / BS_RUN;
     const size_t in_i = i % inner;
     const size_t rest = i / inner;
-    const int k = (int)(rest % n_out_axis);
-    const size_t outer = rest / n_out_axis;
-    const size_t base = (outer * n_in_axis + k) * inner + in_i;
-    double p = 0.0, q = 0.0;
+    const int k0 = (int)(rest % runs_axis) * BS_RUN;
+    const size_t outer = rest / runs_axis;
+    const size_t base = (outer * n_in_axis + k0) * inner + in_i;
+    const size_t obase = (outer * n_out_axis + k0) * inner + in_i;
+    const int nk = min(BS_RUN, n_out_axis - k0);  // outputs of this run
+    const int nin = nk + NB - 1;                  // inputs they cover (k0 + nin <= n_in_axis)
+    double v1[BS_RUN + NB - 1], v2[BS_RUN + NB - 1];
 #pragma unroll
-    for (int j = 0; j < NB; ++j) {
-        p += a1[base + j * step];
-        q += a2[base + j * step];
+    for (int j = 0; j < BS_RUN + NB - 1; ++j) {
+        v1[j] = j < nin ? a1[base + j * step] : 0.0;
+        v2[j] = j < nin ? a2[base + j * step] : 0.0;
     }
-    b1[i] = p;
-    b2[i] = q;
+#pragma unroll
+    for (int k = 0; k < BS_RUN; ++k) {
+        if (k >= nk) break;
+        double p = 0.0, q = 0.0;
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            p += v1[k + j];
+            q += v2[k + j];
+        }
+        b1[obase + k * step] = p;
+        b2[obase + k * step] = q;
+    }
 }
 
 // d_out: 2 * nz * ny * (nx - 15) doubles, receives [W1 | W2] (each (nz-15)(ny-15)(nx-15));
@@ -90,9 +111,12 @@ bool ncc3d_boxsums(const float* d_src, int nx, int ny, int nz, double* d_out, do
     const int wx = nx - NB + 1, wy = ny - NB + 1, wz = nz - NB + 1;
     const size_t na = (size_t)nz * ny * wx, nbq = (size_t)nz * wy * wx, nw = (size_t)wz * wy * wx;
     boxsum_x_kernel<<<(unsigned)((na + 255) / 256), 256, 0, st>>>(d_src, nx, ny, nz, d_out, d_out + na);
-    boxsum_axis_kernel<<<(unsigned)((nbq + 255) / 256), 256, 0, st>>>(d_out, d_out + na, nbq, wx, ny, wy, (size_t)wx,
-                                                                       d_tmp, d_tmp + nbq);
-    boxsum_axis_kernel<<<(unsigned)((nw + 255) / 256), 256, 0, st>>>(d_tmp, d_tmp + nbq, nw, wy * wx, nz, wz,
+    // y: (z, y, x) -> (z, y', x); z: (z, y', x) -> (z', y', x)
+    const size_t ry = (size_t)nz * ((wy + BS_RUN - 1) / BS_RUN) * wx;
+    const size_t rz = (size_t)((wz + BS_RUN - 1) / BS_RUN) * wy * wx;
+    boxsum_axis_kernel<<<(unsigned)((ry + 255) / 256), 256, 0, st>>>(d_out, d_out + na, ry, wx, ny, wy, (size_t)wx,
+                                                                      d_tmp, d_tmp + nbq);
+    boxsum_axis_kernel<<<(unsigned)((rz + 255) / 256), 256, 0, st>>>(d_tmp, d_tmp + nbq, rz, wy * wx, nz, wz,
                                                                       (size_t)wy * wx, d_out, d_out + nw);
     CUT_HIP_CHECK(hipGetLastError());
     return true;
