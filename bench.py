@@ -88,7 +88,10 @@ def motion_leg(dev, stream, seeds=MOTION_SEEDS, repeats=5):
     axis, 2 % decorrelation; one pair per seed): full +-8 ZNCC search vs the
     learned proposal + +-2 refine with full-search fallback, at three peak
     thresholds.  Times are medians per pair; the figures are means over the
-    pairs (agreement also as the minimum)."""
+    pairs (agreement also as the minimum).  Besides agreement with the full
+    search, both are scored against the known rigid shift: the fraction of
+    blocks with full-search peak >= 0.6 whose displacement is within one voxel
+    of it on every axis."""
     import numpy as np
     import torch
 
@@ -114,7 +117,7 @@ def motion_leg(dev, stream, seeds=MOTION_SEEDS, repeats=5):
             t.append(a.elapsed_time(b))
         return float(np.median(t))
 
-    full_ms, confident, shifts = [], [], []
+    full_ms, confident, shifts, full_truth = [], [], [], []
     per_thr = {thr: [] for thr in thresholds}
     nb = 0
     for seed in seeds:
@@ -131,18 +134,26 @@ def motion_leg(dev, stream, seeds=MOTION_SEEDS, repeats=5):
         ms_full = best_ms(lambda: api.ncc3d_search(tpl, src, 8, fd, fp))
         full = fd.cpu().numpy()
         full_ms.append(ms_full)
-        confident.append(float(np.mean(fp.cpu().numpy() >= 0.6)))
+        conf = fp.cpu().numpy() >= 0.6
+        confident.append(float(np.mean(conf)))
+
+        def truth(d):  # confident blocks whose displacement is within 1 voxel (every axis) of the rigid shift
+            return float(np.mean(np.abs(d[conf] - np.asarray(shift)[None, :]).max(axis=1) < 1.0))
+
+        full_truth.append(truth(full))
         for thr in thresholds:
             _, nf = api.motion3d_learned(dp, tpl, src, 8, ld, lp, threshold=thr, count_fallbacks=True)
-            agree = float(np.mean(np.all(ld.cpu().numpy() == full, axis=1)))
+            got = ld.cpu().numpy()
+            agree = float(np.mean(np.all(got == full, axis=1)))
             ms = best_ms(lambda: api.motion3d_learned(dp, tpl, src, 8, ld, lp, threshold=thr))
-            per_thr[thr].append((ms, ms_full / ms, agree, nf / nb))
+            per_thr[thr].append((ms, ms_full / ms, agree, nf / nb, truth(got)))
     mf = float(np.mean(full_ms))
     out = {"workload": f"{len(seeds)} pairs of 128^3 envelope volumes (shifts {shifts} voxels), {nb} blocks of 16^3 "
                        "at stride 8 per pair, +-8 search; means over the pairs",
            "full_ms": round(mf, 4), "full_blocks_per_s": round(nb / (mf * 1e-3)),
            "full_tflops": round(nb * 2 * 4096 * 17 ** 3 / (mf * 1e-3) / 1e12, 2),
-           "confident_fraction": round(float(np.mean(confident)), 4)}
+           "confident_fraction": round(float(np.mean(confident)), 4),
+           "full_within1_of_shift": round(float(np.mean(full_truth)), 4)}
     for thr in thresholds:
         a = np.array(per_thr[thr])
         out[f"learned_thr{thr:.2f}"] = {"ms": round(float(a[:, 0].mean()), 4),
@@ -150,7 +161,8 @@ def motion_leg(dev, stream, seeds=MOTION_SEEDS, repeats=5):
                                         "speedup_vs_full": round(float(a[:, 1].mean()), 3),
                                         "agreement": round(float(a[:, 2].mean()), 4),
                                         "agreement_min": round(float(a[:, 2].min()), 4),
-                                        "fallback_rate": round(float(a[:, 3].mean()), 4)}
+                                        "fallback_rate": round(float(a[:, 3].mean()), 4),
+                                        "within1_of_shift": round(float(a[:, 4].mean()), 4)}
     log(f"bench: motion {json.dumps(out)}")
     return out
 
