@@ -33,7 +33,6 @@ constexpr int NM = 8;            // search margin
 constexpr int NS = 2 * NM + 1;   // shifts per axis
 constexpr int NROW = NB + 2 * NM;
 constexpr int NF_THREADS = 320;  // 17 dz x 17 dy = 289 used
-constexpr int NW_THREADS = 256;
 constexpr int RS = 36;           // LDS row stride (floats) of a staged source plane
 constexpr int TPS = 260;         // LDS plane stride (floats) of the template
 
@@ -318,10 +317,13 @@ __global__ __launch_bounds__(NF_THREADS) void ncc3d_full_kernel(const float* __r
     if (tid == 0) write_result(by_id ? b : (int)blockIdx.x, bi, bv, disp, peak);
 }
 
-// Windowed search |d - centre| <= R (R <= 3).
+// Windowed search |d - centre| <= R (R <= 3).  One thread per (dz, dy) pair and
+// template plane (whole waves: 448 threads at R = 2), so every thread runs one
+// plane and no group of planes waits for a longer one.
 constexpr int NW_MAXR = 3;
+constexpr int nw_threads(int R) { return ((2 * R + 1) * (2 * R + 1) * NB + 63) / 64 * 64; }
 template <int R>
-__global__ __launch_bounds__(NW_THREADS) void ncc3d_window_kernel(const float* __restrict__ tpl,
+__global__ __launch_bounds__(nw_threads(R)) void ncc3d_window_kernel(const float* __restrict__ tpl,
                                                                   const float* __restrict__ src,
                                                                   const double* __restrict__ W1,
                                                                   const double* __restrict__ W2, const N3Geom g,
@@ -329,14 +331,15 @@ __global__ __launch_bounds__(NW_THREADS) void ncc3d_window_kernel(const float* _
                                                                   const int* __restrict__ centre, int by_id,
                                                                   int* __restrict__ disp, float* __restrict__ peak)
 {
+    constexpr int NWT = nw_threads(R);
     constexpr int w = 2 * R + 1, e = NB + 2 * R, pairs = w * w;
-    constexpr int Q = (NW_THREADS / pairs) < NB ? (NW_THREADS / pairs) : NB;
-    static_assert(NW_THREADS * w <= e * e * e, "partial sums reuse the region buffer");
+    constexpr int Q = (NWT / pairs) < NB ? (NWT / pairs) : NB;
+    static_assert(Q * pairs * w <= e * e * e, "partial sums reuse the region buffer");
     __shared__ __attribute__((aligned(16))) float tl[NB * TPS];
     __shared__ __attribute__((aligned(16))) float reg[e * e * e];
-    __shared__ double red[NW_THREADS / 64];
-    __shared__ float rv[NW_THREADS / 64];
-    __shared__ int ri[NW_THREADS / 64];
+    __shared__ double red[NWT / 64];
+    __shared__ float rv[NWT / 64];
+    __shared__ int ri[NWT / 64];
     const int b = ids ? ids[blockIdx.x] : (int)blockIdx.x;
     const int o = by_id ? b : (int)blockIdx.x;
     int bx, by, bz;
@@ -346,7 +349,7 @@ __global__ __launch_bounds__(NW_THREADS) void ncc3d_window_kernel(const float* _
     // region sample 0 sits at source (bx + cx - R, by + cy - R, bz + cz - R); clamped reads
     // only feed shifts that are excluded below
     const int ox = bx + cx - R, oy = by + cy - R, oz = bz + cz - R;
-    for (int k = tid; k < e * e * e; k += NW_THREADS) {
+    for (int k = tid; k < e * e * e; k += NWT) {
         const int X = min(max(ox + k % e, 0), g.nx - 1);
         const int Y = min(max(oy + (k / e) % e, 0), g.ny - 1);
         const int Z = min(max(oz + k / (e * e), 0), g.nz - 1);
@@ -398,7 +401,7 @@ __global__ __launch_bounds__(NW_THREADS) void ncc3d_window_kernel(const float* _
     __syncthreads();
     float bv = -INFINITY;
     int bi = 0x7fffffff;
-    for (int s = tid; s < pairs * w; s += NW_THREADS) {
+    for (int s = tid; s < pairs * w; s += NWT) {
         const int dxr = s % w, p2 = s / w;  // p2 = dzr * w + dyr
         const int dx = cx - R + dxr, dy = cy - R + p2 % w, dz = cz - R + p2 / w;
         if (dx < -NM || dx > NM || dy < -NM || dy > NM || dz < -NM || dz > NM) continue;
@@ -428,10 +431,10 @@ bool launch_ncc3d(const float* d_tpl, const float* d_src, const double* d_w, int
         if (d_count) CUT_FAIL("ncc3d: device block counts apply to the full search only");
         const int bid = by_id ? 1 : 0;
         switch (radius) {
-        case 0: ncc3d_window_kernel<0><<<nblocks, NW_THREADS, 0, st>>>(d_tpl, d_src, W1, W2, g, d_ids, d_centre, bid, d_disp, d_peak); break;
-        case 1: ncc3d_window_kernel<1><<<nblocks, NW_THREADS, 0, st>>>(d_tpl, d_src, W1, W2, g, d_ids, d_centre, bid, d_disp, d_peak); break;
-        case 2: ncc3d_window_kernel<2><<<nblocks, NW_THREADS, 0, st>>>(d_tpl, d_src, W1, W2, g, d_ids, d_centre, bid, d_disp, d_peak); break;
-        default: ncc3d_window_kernel<3><<<nblocks, NW_THREADS, 0, st>>>(d_tpl, d_src, W1, W2, g, d_ids, d_centre, bid, d_disp, d_peak); break;
+        case 0: ncc3d_window_kernel<0><<<nblocks, nw_threads(0), 0, st>>>(d_tpl, d_src, W1, W2, g, d_ids, d_centre, bid, d_disp, d_peak); break;
+        case 1: ncc3d_window_kernel<1><<<nblocks, nw_threads(1), 0, st>>>(d_tpl, d_src, W1, W2, g, d_ids, d_centre, bid, d_disp, d_peak); break;
+        case 2: ncc3d_window_kernel<2><<<nblocks, nw_threads(2), 0, st>>>(d_tpl, d_src, W1, W2, g, d_ids, d_centre, bid, d_disp, d_peak); break;
+        default: ncc3d_window_kernel<3><<<nblocks, nw_threads(3), 0, st>>>(d_tpl, d_src, W1, W2, g, d_ids, d_centre, bid, d_disp, d_peak); break;
         }
     } else {
         ncc3d_full_kernel<<<nblocks, NF_THREADS, 0, st>>>(d_tpl, d_src, W1, W2, g, d_ids, d_count, by_id ? 1 : 0,
