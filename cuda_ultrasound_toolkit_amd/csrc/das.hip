@@ -267,6 +267,21 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
             base_d[j] = uni_d(base_d[j]);
         }
     }
+    // Smallest and largest (fn/2z)^2 over the tile's depths (the deepest and
+    // the shallowest when the tile lies below the array, z > 0), and from them
+    // two radii for rho^2, uniform over the tile: r2_any, below which some
+    // depth of a column can be inside the aperture (0.2 % margin), and
+    // r2_full, below which every depth is, in the same FP32 product
+    // rho2 * kz2[j] < HANN_S_THR that the taps test (rho2 < r2_full gives
+    // rho2 * kz2_hi <= r2_full * kz2_hi < HANN_S_THR, products being monotonic).
+    float kz2_lo = kz2[0], kz2_hi = kz2[0];
+#pragma unroll
+    for (int j = 1; j < HT_Z; ++j) { kz2_lo = fminf(kz2_lo, kz2[j]); kz2_hi = fmaxf(kz2_hi, kz2[j]); }
+    constexpr float HANN_S_THR_W = (HANN_A_THR * 1.001f) * (HANN_A_THR * 1.001f);
+    const float r2_any = uni(HANN_S_THR_W / kz2_lo);
+    float r2_full = HANN_S_THR / kz2_hi;
+    while (r2_full > 0.0f && !(r2_full * kz2_hi < HANN_S_THR)) r2_full = __int_as_float(__float_as_int(r2_full) - 1);
+    r2_full = uni(r2_full);
     // transmit path bounds over the tile
     float txd_lo, txd_hi;
     if constexpr (FOCUS) {
@@ -419,9 +434,8 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
                         apo_c = sin_apo_to(ex, p.xdc_max_x, p.to_power) * apo_y;
                         return apo_c > 0.1f;  // uniform: depends on the element only
                     } else {
-                        // the deepest voxel of the column has the widest aperture
-                        constexpr float thr = (HANN_A_THR * 1.001f) * (HANN_A_THR * 1.001f);
-                        return unbounded || __builtin_amdgcn_ballot_w64(rho2 * kz2[HT_Z - 1] < thr) != 0;
+                        // the depth with the smallest (fn/2z)^2 has the widest aperture
+                        return unbounded || __builtin_amdgcn_ballot_w64(rho2 * kz2_lo < HANN_S_THR_W) != 0;
                     }
                 };
                 // Sample indices of this block of channels are taken relative to
@@ -448,55 +462,131 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
                 float base_r[HT_Z];
 #pragma unroll
                 for (int j = 0; j < HT_Z; ++j) base_r[j] = (float)(base_d[j] - (double)org);
-                // channels whose window sits in LDS; a single path keeps the
-                // accumulators in place across iterations
-                for (int cc = 0; cc < nC; ++cc) {
-                    if (!((lds_mask >> cc) & 1ull)) continue;
-                    const int wb = __builtin_amdgcn_readlane(my_base, cc);
-                    float rho2, apo_c;
-                    if (!chan(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(my_ex), cc)), rho2, apo_c))
-                        continue;
-                    const char* wbytes = reinterpret_cast<const char*>(win);
-                    // one tap: sample index, window entry, interpolation, accumulation
-                    auto tap_at = [&](int j, float w) {
-                        // the window covers every voxel of the tile with a 2-sample margin,
-                        // so the index needs no clamp beyond the reference's [1, S-2], and
-                        // none at all in tiles whose every index lies inside it
-                        const float xr = fmaf(fsqrt(rho2 + z2[j]), p.spm, base_r[j]);
-                        const float x = CLAMP ? __builtin_amdgcn_fmed3f(xr, lo_r, hi_r) : xr;
-                        const float tt = __builtin_amdgcn_fractf(x);
-                        const int addr = wb + (int)x * (int)WIN_ESZ;  // window entry of block-relative index
-                        float2 v;
-                        if constexpr (CUBIC) {
-                            const float4 e0 = *reinterpret_cast<const float4*>(wbytes + addr);
-                            const float4 e1 = *reinterpret_cast<const float4*>(wbytes + addr + HPOOL * WIN_ESZ);
-                            v = make_float2(fmaf(fmaf(fmaf(e1.z, tt, e1.x), tt, e0.z), tt, e0.x),
-                                            fmaf(fmaf(fmaf(e1.w, tt, e1.y), tt, e0.w), tt, e0.y));
-                        } else {
-                            const float4 e = *reinterpret_cast<const float4*>(wbytes + addr);
-                            v = make_float2(fmaf(tt, e.z, e.x), fmaf(tt, e.w, e.y));
-                        }
-                        acc[j].x = fmaf(w, v.x, acc[j].x);
-                        acc[j].y = fmaf(w, v.y, acc[j].y);
-                        if constexpr (CF) inc[j] = fmaf(v.x, v.x, fmaf(v.y, v.y, inc[j]));  // w != 0 here
-                    };
-                    // Taps outside the aperture get weight 0, or, with the coherence
-                    // factor, are masked off (and skipped when no lane of the wave is
-                    // inside), which measured faster there (and slower without it).
-                    // chan() already required apo_c > 0.1.
+                const char* wbytes = reinterpret_cast<const char*>(win);
+                // one tap of channel cc (LDS window base wb, lateral distance^2
+                // rho2): sample index, window entry, interpolation, accumulation
+                auto tap_at = [&](int j, float w, int wb, float rho2) {
+                    // the window covers every voxel of the tile with a 2-sample margin,
+                    // so the index needs no clamp beyond the reference's [1, S-2], and
+                    // none at all in tiles whose every index lies inside it
+                    const float xr = fmaf(fsqrt(rho2 + z2[j]), p.spm, base_r[j]);
+                    const float x = CLAMP ? __builtin_amdgcn_fmed3f(xr, lo_r, hi_r) : xr;
+                    const float tt = __builtin_amdgcn_fractf(x);
+                    const int addr = wb + (int)x * (int)WIN_ESZ;  // window entry of block-relative index
+                    float2 v;
+                    if constexpr (CUBIC) {
+                        const float4 e0 = *reinterpret_cast<const float4*>(wbytes + addr);
+                        const float4 e1 = *reinterpret_cast<const float4*>(wbytes + addr + HPOOL * WIN_ESZ);
+                        v = make_float2(fmaf(fmaf(fmaf(e1.z, tt, e1.x), tt, e0.z), tt, e0.x),
+                                        fmaf(fmaf(fmaf(e1.w, tt, e1.y), tt, e0.w), tt, e0.y));
+                    } else {
+                        const float4 e = *reinterpret_cast<const float4*>(wbytes + addr);
+                        v = make_float2(fmaf(tt, e.z, e.x), fmaf(tt, e.w, e.y));
+                    }
+                    acc[j].x = fmaf(w, v.x, acc[j].x);
+                    acc[j].y = fmaf(w, v.y, acc[j].y);
+                    if constexpr (CF) inc[j] = fmaf(v.x, v.x, fmaf(v.y, v.y, inc[j]));  // w != 0 here
+                };
+                auto chan_ex = [&](int cc) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(my_ex), cc)); };
+                if constexpr (!SINAPO && !CF) {
+                    // Hann apodization without the coherence factor.  The pass's
+                    // channels are sorted first into two scalar masks: those where
+                    // every lane of the wave is inside the aperture at every depth of
+                    // the tile (rho2 < r2_full; most wave-channels of a volume), whose
+                    // taps need no threshold test, and those where only some are
+                    // (rho2 < r2_any for some lane).  Two plain loops then keep the
+                    // accumulators in place (a branch between the two tap forms inside
+                    // one loop measured slower: the compiler moves the accumulators
+                    // between register sets at its join).
+                    unsigned long long m_full = 0, m_part = 0;
+                    for (int cc = 0; cc < nC; ++cc) {
+                        if (!((lds_mask >> cc) & 1ull)) continue;
+                        const float dx = chan_ex(cc) - vx;
+                        const float rho2 = fmaf(dx, dx, dy2);
+                        if (!unbounded && __builtin_amdgcn_ballot_w64(rho2 < r2_any) == 0ull) continue;
+                        if (__builtin_amdgcn_ballot_w64(!(rho2 < r2_full)) == 0ull) m_full |= 1ull << cc;
+                        else m_part |= 1ull << cc;
+                    }
+                    // The 8 taps of a channel are software-pipelined: the window
+                    // entry (entries, cubic) of depth j+1 is requested before depth j
+                    // is accumulated, so each wave keeps LDS reads in flight across
+                    // its arithmetic (each tap waiting on its own read left the
+                    // kernel latency-bound at four waves per SIMD).
+                    auto taps8 = [&](int wb, float rho2, auto thr) {
+                        struct Ent {
+                            float4 e0, e1;
+                            float tt;
+                        };
+                        auto fetch = [&](int j) {
+                            Ent r;
+                            const float xr = fmaf(fsqrt(rho2 + z2[j]), p.spm, base_r[j]);
+                            const float x = CLAMP ? __builtin_amdgcn_fmed3f(xr, lo_r, hi_r) : xr;
+                            r.tt = __builtin_amdgcn_fractf(x);
+                            const int addr = wb + (int)x * (int)WIN_ESZ;
+                            r.e0 = *reinterpret_cast<const float4*>(wbytes + addr);
+                            if constexpr (CUBIC) r.e1 = *reinterpret_cast<const float4*>(wbytes + addr + HPOOL * WIN_ESZ);
+                            return r;
+                        };
+                        Ent cur = fetch(0);
 #pragma unroll
-                    for (int j = 0; j < HT_Z; ++j) {
-                        float w;
-                        if constexpr (SINAPO) {
-                            w = apo_c;
-                        } else if constexpr (CF) {
-                            const float s = rho2 * kz2[j];
-                            if (!(s < HANN_S_THR)) continue;
-                            w = hann_poly(s);
-                        } else {
-                            w = tap_weight<false>(rho2, kz2[j], apo_c);
+                        for (int j = 0; j < HT_Z; ++j) {
+                            Ent nxt;
+                            if (j + 1 < HT_Z) nxt = fetch(j + 1);
+                            float w;
+                            if constexpr (decltype(thr)::value) w = tap_weight<false>(rho2, kz2[j], 0.0f);
+                            else w = hann_poly(rho2 * kz2[j]);
+                            const float tt = cur.tt;
+                            float2 v;
+                            if constexpr (CUBIC)
+                                v = make_float2(fmaf(fmaf(fmaf(cur.e1.z, tt, cur.e1.x), tt, cur.e0.z), tt, cur.e0.x),
+                                                fmaf(fmaf(fmaf(cur.e1.w, tt, cur.e1.y), tt, cur.e0.w), tt, cur.e0.y));
+                            else
+                                v = make_float2(fmaf(tt, cur.e0.z, cur.e0.x), fmaf(tt, cur.e0.w, cur.e0.y));
+                            acc[j].x = fmaf(w, v.x, acc[j].x);
+                            acc[j].y = fmaf(w, v.y, acc[j].y);
+                            if (j + 1 < HT_Z) cur = nxt;
                         }
-                        tap_at(j, w);
+                    };
+                    while (m_full) {
+                        const int cc = __builtin_ctzll(m_full);
+                        m_full &= m_full - 1;
+                        const int wb = __builtin_amdgcn_readlane(my_base, cc);
+                        const float dx = chan_ex(cc) - vx;
+                        taps8(wb, fmaf(dx, dx, dy2), std::false_type{});
+                    }
+                    while (m_part) {
+                        const int cc = __builtin_ctzll(m_part);
+                        m_part &= m_part - 1;
+                        const int wb = __builtin_amdgcn_readlane(my_base, cc);
+                        const float dx = chan_ex(cc) - vx;
+                        taps8(wb, fmaf(dx, dx, dy2), std::true_type{});
+                    }
+                } else {
+                    // channels whose window sits in LDS; a single path keeps the
+                    // accumulators in place across iterations
+                    for (int cc = 0; cc < nC; ++cc) {
+                        if (!((lds_mask >> cc) & 1ull)) continue;
+                        const int wb = __builtin_amdgcn_readlane(my_base, cc);
+                        float rho2, apo_c;
+                        if (!chan(chan_ex(cc), rho2, apo_c)) continue;
+                        // Taps outside the aperture get weight 0, or, with the coherence
+                        // factor, are masked off (and skipped when no lane of the wave is
+                        // inside), which measured faster there (and slower without it).
+                        // chan() already required apo_c > 0.1.
+#pragma unroll
+                        for (int j = 0; j < HT_Z; ++j) {
+                            float w;
+                            if constexpr (SINAPO) {
+                                w = apo_c;
+                            } else if constexpr (CF) {
+                                const float s = rho2 * kz2[j];
+                                if (!(s < HANN_S_THR)) continue;
+                                w = hann_poly(s);
+                            } else {
+                                w = tap_weight<false>(rho2, kz2[j], apo_c);
+                            }
+                            tap_at(j, w, wb, rho2);
+                        }
                     }
                 }
                 // channels whose window did not fit: direct reads
