@@ -215,6 +215,12 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
     const double vy_d = (double)p.min_y + (double)min(iy, p.Ny - 1) * (double)p.res_y;
     const float vx = (float)((double)p.min_x + (double)min(ix, p.Nx - 1) * (double)p.res_x);
     const float vy = (float)vy_d;
+    // the wave's patch corners: lane 0 (first column, first row), lane 7 (last
+    // column) and lane 56 (last row), the same FP32 values the lanes hold
+    const float wvx0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(vx), 0));
+    const float wvx1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(vx), 7));
+    const float wvy0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(vy), 0));
+    const float wvy1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(vy), 56));
 
     // tile bounding box
     const float xa = p.min_x + (float)bx * p.res_x, xb = p.min_x + (float)min(bx + HT_X - 1, p.Nx - 1) * p.res_x;
@@ -342,6 +348,12 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
             }
             const float dy = ey - vy;
             const float dy2 = dy * dy;
+            // largest dy^2 of the wave's lanes (one of its two edge rows, the same
+            // product), and a lower bound of the smallest
+            const float dya = ey - wvy0, dyb = ey - wvy1;
+            const float wdy2_hi = fmaxf(dya * dya, dyb * dyb);
+            const bool ey_in = ey >= fminf(wvy0, wvy1) && ey <= fmaxf(wvy0, wvy1);
+            const float wdy2_lo = ey_in ? 0.0f : fminf(dya * dya, dyb * dyb);
             const size_t toff = (size_t)t * p.C * rowlen;
             if (neg) flip();
             float apo_y = 1.0f;
@@ -498,14 +510,21 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
                     // accumulators in place (a branch between the two tap forms inside
                     // one loop measured slower: the compiler moves the accumulators
                     // between register sets at its join).
-                    unsigned long long m_full = 0, m_part = 0;
-                    for (int cc = 0; cc < nC; ++cc) {
-                        if (!((lds_mask >> cc) & 1ull)) continue;
-                        const float dx = chan_ex(cc) - vx;
-                        const float rho2 = fmaf(dx, dx, dy2);
-                        if (!unbounded && __builtin_amdgcn_ballot_w64(rho2 < r2_any) == 0ull) continue;
-                        if (__builtin_amdgcn_ballot_w64(!(rho2 < r2_full)) == 0ull) m_full |= 1ull << cc;
-                        else m_part |= 1ull << cc;
+                    // Lane cc sorts channel cc for the whole wave from the patch's
+                    // extreme lanes: the largest rho2 of any lane is that of a corner
+                    // lane (the largest |dx| and dy^2, same FP32 expression, monotonic),
+                    // and the patch box bounds the smallest from below.
+                    unsigned long long m_full, m_part;
+                    {
+                        const float dxa = my_ex - wvx0, dxb = my_ex - wvx1;
+                        const float dx_hi = fmaxf(fabsf(dxa), fabsf(dxb));
+                        const bool ex_in = my_ex >= fminf(wvx0, wvx1) && my_ex <= fmaxf(wvx0, wvx1);
+                        const float dx_lo = ex_in ? 0.0f : fminf(fabsf(dxa), fabsf(dxb));
+                        const bool any_in = unbounded || fmaf(dx_lo, dx_lo, wdy2_lo) < r2_any;
+                        const bool all_in = fmaf(dx_hi, dx_hi, wdy2_hi) < r2_full;
+                        const unsigned long long m_any = __builtin_amdgcn_ballot_w64(any_in) & lds_mask;
+                        m_full = __builtin_amdgcn_ballot_w64(all_in) & m_any;
+                        m_part = m_any & ~m_full;
                     }
                     // The 8 taps of a channel are software-pipelined: the window
                     // entry (entries, cubic) of depth j+1 is requested before depth j
