@@ -34,6 +34,7 @@ namespace cut {
 
 constexpr int HT_X = 16, HT_Y = 16, HT_Z = 8;  // voxel tile
 constexpr int HCB = 16;                         // most channels staged per pass
+constexpr int HSB = 64;                         // channels per block of window tables (one per lane)
 constexpr int HW = 256;                         // longest window staged in LDS (samples)
 // LDS window pool of a workgroup: 32 KiB, shared by the channels of a pass at
 // offsets given by a prefix sum of their window lengths.  Each sample i of a
@@ -191,7 +192,7 @@ template <bool CUBIC, bool FOCUS, bool CF, bool SINAPO, int IQK, bool CLAMP>
 __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float iq_inv, const DasParams& p, int bx,
                                               int by, int bz, int gt_lo, int gt_hi, float2* __restrict__ vol,
                                               void* __restrict__ part, float4* win, int* wlo_s, int* wmode_s,
-                                              int* woff_s)
+                                              int* woff_s, int* wlen_s)
 {
     // window entry stride (bytes), pool capacity (entries), channels per pass
     // and staging threads per channel.  Cubic entries are split into two
@@ -359,47 +360,81 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
             float apo_y = 1.0f;
             if constexpr (SINAPO) apo_y = sin_apo_to(ey, p.xdc_max_y, p.to_power);
 
-            for (int cb = c_lo; cb <= c_hi; cb += NCB) {
-                const int nC = min(NCB, c_hi - cb + 1);
-                __syncthreads();
-                {
-                    // Lane k of every NCB-lane group bounds the window of channel
-                    // cb + k (the sample span any voxel of the tile can read); an
-                    // exclusive prefix sum over the group places the windows in
-                    // the pool.  Threads [wi*TPC, (wi+1)*TPC) then stage channel cb + wi.
-                    const int kb = tid & (NCB - 1), wi = tid / TPC, part = tid % TPC;
-                    int wlo = 0, len = 0;
-                    if (kb < nC) {
-                        const int c = cb + kb;
-                        const float ex = fmaf((float)c, p.pitch_x, p.xdc_min_x);
-                        const MinMax dxr = dist_range(ex, X0, X1);
-                        // v_sqrt_f32 (1 ulp, ~1e-4 samples here) against the windows'
-                        // 2-sample margins: the IEEE sqrtf expansion is not needed
-                        const float rmin = fsqrt(dxr.lo * dxr.lo + dyr.lo * dyr.lo + zr.lo * zr.lo);
-                        const float rmax = fsqrt(dxr.hi * dxr.hi + dyr.hi * dyr.hi + zr.hi * zr.hi);
-                        float ilo = fmaf(txd_lo + rmin + fz, p.spm, (float)p.delay);
-                        float ihi = fmaf(txd_hi + rmax + fz, p.spm, (float)p.delay);
-                        if (p.spm < 0.0f) { const float tt = ilo; ilo = ihi; ihi = tt; }
-                        ilo = fminf(fmaxf(ilo, lo_clamp), hi_clamp);
-                        ihi = fminf(fmaxf(ihi, lo_clamp), hi_clamp);
-                        wlo = max(0, (int)floorf(ilo) - 2);
-                        const int whi = min(p.S - 1, (int)floorf(ihi) + 3);
-                        len = whi - wlo + 1;
-                    }
-                    const bool fits = kb < nC && len > 0 && len <= HW;
-                    const int need = fits ? len : 0;
-                    int scan = need;
+            // Channels go in blocks of up to 64.  Wave 0 bounds the windows of a
+            // block's channels, one per lane (the sample span any voxel of the
+            // tile can read), and places them in the LDS pool pass by pass (an
+            // exclusive prefix sum over each NCB-lane group; a pass is NCB
+            // channels): one computation for a block instead of one per wave
+            // and pass.  Each wave meanwhile sorts the block's channels for its
+            // own patch (below).
+            for (int cb64 = c_lo; cb64 <= c_hi; cb64 += HSB) {
+              const int nC64 = min(HSB, c_hi - cb64 + 1);
+              const int lane = tid & 63;
+              __syncthreads();  // the previous block is done with the tables and the pool
+              if (tid < 64) {
+                  const int kb = lane & (NCB - 1);
+                  int wlo = 0, len = 0;
+                  if (lane < nC64) {
+                      const int c = cb64 + lane;
+                      const float ex = fmaf((float)c, p.pitch_x, p.xdc_min_x);
+                      const MinMax dxr = dist_range(ex, X0, X1);
+                      // v_sqrt_f32 (1 ulp, ~1e-4 samples here) against the windows'
+                      // 2-sample margins: the IEEE sqrtf expansion is not needed
+                      const float rmin = fsqrt(dxr.lo * dxr.lo + dyr.lo * dyr.lo + zr.lo * zr.lo);
+                      const float rmax = fsqrt(dxr.hi * dxr.hi + dyr.hi * dyr.hi + zr.hi * zr.hi);
+                      float ilo = fmaf(txd_lo + rmin + fz, p.spm, (float)p.delay);
+                      float ihi = fmaf(txd_hi + rmax + fz, p.spm, (float)p.delay);
+                      if (p.spm < 0.0f) { const float tt = ilo; ilo = ihi; ihi = tt; }
+                      ilo = fminf(fmaxf(ilo, lo_clamp), hi_clamp);
+                      ihi = fminf(fmaxf(ihi, lo_clamp), hi_clamp);
+                      wlo = max(0, (int)floorf(ilo) - 2);
+                      // the linear entry of sample i holds p[i + 1] - p[i], so a window
+                      // ends at S-2 (indices are clamped to [1, S-2]; entry S-1 is never
+                      // read); the cubic entries clamp their neighbours themselves
+                      const int whi = min(CUBIC ? p.S - 1 : p.S - 2, (int)floorf(ihi) + 3);
+                      len = whi - wlo + 1;
+                  }
+                  const bool fits = lane < nC64 && len > 0 && len <= HW;
+                  const int need = fits ? len : 0;
+                  int scan = need;
 #pragma unroll
-                    for (int off = 1; off < NCB; off <<= 1) {
-                        const int v = __shfl_up(scan, off, NCB);
-                        if (kb >= off) scan += v;
-                    }
-                    const int mode = (fits && scan <= HPOOL) ? 0 : 1;
-                    const int c_wlo = __shfl(wlo, wi, NCB), c_len = __shfl(len, wi, NCB);
-                    const int c_off = __shfl(scan - need, wi, NCB), c_mode = __shfl(mode, wi, NCB);
-                    if (wi < nC) {
-                        if (part == 0) { wlo_s[wi] = c_wlo; wmode_s[wi] = c_mode; woff_s[wi] = c_off; }
-                        if (c_mode == 0) {
+                  for (int off = 1; off < NCB; off <<= 1) {
+                      const int v = __shfl_up(scan, off, NCB);
+                      if (kb >= off) scan += v;
+                  }
+                  wlo_s[lane] = wlo;
+                  wlen_s[lane] = len;
+                  woff_s[lane] = scan - need;
+                  wmode_s[lane] = (fits && scan <= HPOOL) ? 0 : 1;
+              }
+              // lane L holds the element x of channel cb64 + L
+              const float my_ex = fmaf((float)(cb64 + lane), p.pitch_x, p.xdc_min_x);
+              // Lane L sorts channel cb64 + L for the whole wave from the patch's
+              // extreme lanes: the largest rho2 of any lane is that of a corner lane
+              // (the largest |dx| and dy^2, same FP32 expression, monotonic), and
+              // the patch box bounds the smallest from below.
+              unsigned long long m_any64 = 0, m_full64 = 0;
+              if constexpr (!SINAPO && !CF) {
+                  const float dxa = my_ex - wvx0, dxb = my_ex - wvx1;
+                  const float dx_hi = fmaxf(fabsf(dxa), fabsf(dxb));
+                  const bool ex_in = my_ex >= fminf(wvx0, wvx1) && my_ex <= fmaxf(wvx0, wvx1);
+                  const float dx_lo = ex_in ? 0.0f : fminf(fabsf(dxa), fabsf(dxb));
+                  const bool any_in = lane < nC64 && (unbounded || fmaf(dx_lo, dx_lo, wdy2_lo) < r2_any);
+                  const bool all_in = fmaf(dx_hi, dx_hi, wdy2_hi) < r2_full;
+                  m_any64 = __builtin_amdgcn_ballot_w64(any_in);
+                  m_full64 = __builtin_amdgcn_ballot_w64(all_in) & m_any64;
+              }
+              __syncthreads();  // tables ready
+              for (int sub = 0; sub < nC64; sub += NCB) {
+                const int cb = cb64 + sub;
+                const int nC = min(NCB, nC64 - sub);
+                if (sub > 0) __syncthreads();  // the previous pass is done with the pool
+                {
+                    // threads [wi*TPC, (wi+1)*TPC) stage channel cb + wi
+                    const int wi = tid / TPC, part = tid % TPC;
+                    if (wi < nC && wmode_s[sub + wi] == 0) {
+                        const int c_wlo = wlo_s[sub + wi], c_len = wlen_s[sub + wi], c_off = woff_s[sub + wi];
+                        {
                             const size_t row = toff + (size_t)(cb + wi) * rowlen;
                             if constexpr (CUBIC) {
                                 // Catmull-Rom on [i, i+1] (tangents (p[i+1] - p[i-1]) / 2):
@@ -423,12 +458,12 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
                                     w4[HPOOL + q] = make_float4(c2x, c2y, c3x, c3y);
                                 }
                             } else {
-                                // (p[i], p[i+1] - p[i]); the entry of sample S-1 is never read
+                                // (p[i], p[i+1] - p[i]), i <= S-2: both samples from one address
                                 float4* w4 = win + c_off;
                                 for (int q = part; q < c_len; q += TPC) {
-                                    const int i = c_wlo + q;
-                                    const float2 a = iq_at<IQK>(iq, row + i, iq_inv);
-                                    const float2 b = iq_at<IQK>(iq, row + min(i + 1, p.S - 1), iq_inv);
+                                    const size_t i = row + (size_t)(c_wlo + q);
+                                    const float2 a = iq_at<IQK>(iq, i, iq_inv);
+                                    const float2 b = iq_at<IQK>(iq, i + 1, iq_inv);
                                     w4[q] = make_float4(a.x, a.y, b.x - a.x, b.y - a.y);
                                 }
                             }
@@ -459,16 +494,14 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
                 // cc's element x and LDS base, bit cc of lds_mask whether its
                 // window is in LDS; the channel loops read them with readlane, so
                 // their bookkeeping stays in scalar registers.
-                const int lane = tid & 63;
-                const bool my_lds = lane < nC && wmode_s[lane] == 0;
-                const int my_wlo = my_lds ? wlo_s[lane] : 0x7fffffff;
+                const bool my_lds = lane < nC && wmode_s[sub + lane] == 0;
+                const int my_wlo = my_lds ? wlo_s[sub + lane] : 0x7fffffff;
                 const unsigned long long lds_mask = __builtin_amdgcn_ballot_w64(my_lds);
                 int org = my_wlo;
 #pragma unroll
                 for (int off = 8; off > 0; off >>= 1) org = min(org, __shfl_xor(org, off, 16));
                 org = __builtin_amdgcn_readfirstlane(org);
-                const int my_base = my_lds ? (woff_s[lane] - (my_wlo - org)) * (int)WIN_ESZ : 0;
-                const float my_ex = fmaf((float)(cb + lane), p.pitch_x, p.xdc_min_x);
+                const int my_base = my_lds ? (woff_s[sub + lane] - (my_wlo - org)) * (int)WIN_ESZ : 0;
                 const float forg = (float)org;
                 const float lo_r = lo_clamp - forg, hi_r = hi_clamp - forg;
                 float base_r[HT_Z];
@@ -499,7 +532,7 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
                     acc[j].y = fmaf(w, v.y, acc[j].y);
                     if constexpr (CF) inc[j] = fmaf(v.x, v.x, fmaf(v.y, v.y, inc[j]));  // w != 0 here
                 };
-                auto chan_ex = [&](int cc) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(my_ex), cc)); };
+                auto chan_ex = [&](int cc) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(my_ex), sub + cc)); };
                 if constexpr (!SINAPO && !CF) {
                     // Hann apodization without the coherence factor.  The pass's
                     // channels are sorted first into two scalar masks: those where
@@ -510,22 +543,9 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
                     // accumulators in place (a branch between the two tap forms inside
                     // one loop measured slower: the compiler moves the accumulators
                     // between register sets at its join).
-                    // Lane cc sorts channel cc for the whole wave from the patch's
-                    // extreme lanes: the largest rho2 of any lane is that of a corner
-                    // lane (the largest |dx| and dy^2, same FP32 expression, monotonic),
-                    // and the patch box bounds the smallest from below.
-                    unsigned long long m_full, m_part;
-                    {
-                        const float dxa = my_ex - wvx0, dxb = my_ex - wvx1;
-                        const float dx_hi = fmaxf(fabsf(dxa), fabsf(dxb));
-                        const bool ex_in = my_ex >= fminf(wvx0, wvx1) && my_ex <= fmaxf(wvx0, wvx1);
-                        const float dx_lo = ex_in ? 0.0f : fminf(fabsf(dxa), fabsf(dxb));
-                        const bool any_in = unbounded || fmaf(dx_lo, dx_lo, wdy2_lo) < r2_any;
-                        const bool all_in = fmaf(dx_hi, dx_hi, wdy2_hi) < r2_full;
-                        const unsigned long long m_any = __builtin_amdgcn_ballot_w64(any_in) & lds_mask;
-                        m_full = __builtin_amdgcn_ballot_w64(all_in) & m_any;
-                        m_part = m_any & ~m_full;
-                    }
+                    const unsigned long long m_any = (m_any64 >> sub) & lds_mask;
+                    unsigned long long m_full = (m_full64 >> sub) & m_any;
+                    unsigned long long m_part = m_any & ~m_full;
                     // The 8 taps of a channel are software-pipelined: the window
                     // entry (entries, cubic) of depth j+1 is requested before depth j
                     // is accumulated, so each wave keeps LDS reads in flight across
@@ -612,7 +632,7 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
                 for (int cc = 0; cc < nC; ++cc) {
                     if ((lds_mask >> cc) & 1ull) continue;
                     float rho2, apo_c;
-                    if (!chan(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(my_ex), cc)), rho2, apo_c))
+                    if (!chan(chan_ex(cc), rho2, apo_c))
                         continue;
                     const size_t grow = toff + (size_t)(cb + cc) * rowlen;
 #pragma unroll
@@ -626,6 +646,7 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
                         if constexpr (CF) inc[j] = fmaf(v.x, v.x, fmaf(v.y, v.y, inc[j]));
                     }
                 }
+              }
             }
             if (neg) flip();
         }
@@ -706,9 +727,10 @@ __global__ __launch_bounds__(256, 4) void hercules_kernel(const void* __restrict
                                                        void* __restrict__ part)
 {
     __shared__ float4 win[HPOOL_BYTES / sizeof(float4)];
-    __shared__ int wlo_s[HCB];
-    __shared__ int wmode_s[HCB];
-    __shared__ int woff_s[HCB];
+    __shared__ int wlo_s[HSB];
+    __shared__ int wmode_s[HSB];
+    __shared__ int woff_s[HSB];
+    __shared__ int wlen_s[HSB];
     const int per_z = hg.nxt * hg.nyt * hg.splits;
     const int wz = blockIdx.x / per_z;
     const int wr = blockIdx.x - wz * per_z;
@@ -722,10 +744,10 @@ __global__ __launch_bounds__(256, 4) void hercules_kernel(const void* __restrict
     void* pp = hg.splits > 1 ? static_cast<char*>(part) + (size_t)split * p.Nx * p.Ny * p.Nz * PART_ESZ : nullptr;
     if (tile_never_clamps<FOCUS>(p, bx, by, bz))
         hercules_tile<CUBIC, FOCUS, CF, SINAPO, IQK, false>(iq, iq_inv, p, bx, by, bz, gt_lo, gt_hi, vol, pp, win,
-                                                            wlo_s, wmode_s, woff_s);
+                                                            wlo_s, wmode_s, woff_s, wlen_s);
     else
         hercules_tile<CUBIC, FOCUS, CF, SINAPO, IQK, true>(iq, iq_inv, p, bx, by, bz, gt_lo, gt_hi, vol, pp, win,
-                                                           wlo_s, wmode_s, woff_s);
+                                                           wlo_s, wmode_s, woff_s, wlen_s);
 }
 
 // A batch of frames of any sizes in one launch.  Work items (frame, tile) are
@@ -741,9 +763,10 @@ __global__ __launch_bounds__(256, 4) void hercules_batch_kernel(const BatchFrame
                                                              long long* __restrict__ finish)
 {
     __shared__ float4 win[HPOOL_BYTES / sizeof(float4)];
-    __shared__ int wlo_s[HCB];
-    __shared__ int wmode_s[HCB];
-    __shared__ int woff_s[HCB];
+    __shared__ int wlo_s[HSB];
+    __shared__ int wmode_s[HSB];
+    __shared__ int woff_s[HSB];
+    __shared__ int wlen_s[HSB];
     __shared__ int item_s;
     const bool queue = gridDim.x < (unsigned)n_items;
     int it = blockIdx.x;
@@ -765,10 +788,10 @@ __global__ __launch_bounds__(256, 4) void hercules_batch_kernel(const BatchFrame
         const int G = p.readi_groups > 1 ? p.readi_groups : 1;
         if (tile_never_clamps<FOCUS>(p, bx, by, bz))
             hercules_tile<CUBIC, FOCUS, CF, SINAPO, IQK, false>(fr.iq, fr.iq_inv, p, bx, by, bz, 0, G * p.T, fr.vol,
-                                                                nullptr, win, wlo_s, wmode_s, woff_s);
+                                                                nullptr, win, wlo_s, wmode_s, woff_s, wlen_s);
         else
             hercules_tile<CUBIC, FOCUS, CF, SINAPO, IQK, true>(fr.iq, fr.iq_inv, p, bx, by, bz, 0, G * p.T, fr.vol,
-                                                               nullptr, win, wlo_s, wmode_s, woff_s);
+                                                               nullptr, win, wlo_s, wmode_s, woff_s, wlen_s);
         __syncthreads();
         if (threadIdx.x == 0) {
             __threadfence();
