@@ -458,13 +458,27 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
                                     w4[HPOOL + q] = make_float4(c2x, c2y, c3x, c3y);
                                 }
                             } else {
-                                // (p[i], p[i+1] - p[i]), i <= S-2: both samples from one address
+                                // (p[i], p[i+1] - p[i]), i <= S-2: both samples from one address.
+                                // Four entries per thread are requested before any is written, so a
+                                // window's loads overlap instead of each waiting on the last (when
+                                // the rows come from HBM rather than L2 the wait would otherwise
+                                // exceed a pass's arithmetic).
                                 float4* w4 = win + c_off;
-                                for (int q = part; q < c_len; q += TPC) {
-                                    const size_t i = row + (size_t)(c_wlo + q);
-                                    const float2 a = iq_at<IQK>(iq, i, iq_inv);
-                                    const float2 b = iq_at<IQK>(iq, i + 1, iq_inv);
-                                    w4[q] = make_float4(a.x, a.y, b.x - a.x, b.y - a.y);
+                                constexpr int U = 4;
+                                for (int q0 = part; q0 < c_len; q0 += U * TPC) {
+                                    float2 a[U], b[U];
+#pragma unroll
+                                    for (int u = 0; u < U; ++u) {
+                                        const int q = min(q0 + u * TPC, c_len - 1);
+                                        const size_t i = row + (size_t)(c_wlo + q);
+                                        a[u] = iq_at<IQK>(iq, i, iq_inv);
+                                        b[u] = iq_at<IQK>(iq, i + 1, iq_inv);
+                                    }
+#pragma unroll
+                                    for (int u = 0; u < U; ++u) {
+                                        const int q = q0 + u * TPC;
+                                        if (q < c_len) w4[q] = make_float4(a[u].x, a[u].y, b[u].x - a[u].x, b[u].y - a[u].y);
+                                    }
                                 }
                             }
                         }
