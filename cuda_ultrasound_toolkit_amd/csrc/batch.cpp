@@ -201,12 +201,13 @@ extern "C" bool cut_beamform_batch(const CutFrame* frames, int n, int mode, doub
 
         // graph mode: HERCULES frames sharing one kernel variant
         const cut::FramePlan& p0 = s.slots[0]->plan;
-        const bool focus = p0.p.focus_type == 4, sinapo = p0.p.apo_type == RX_TO_SIN;
+        const bool focus = p0.p.focus_type == 4, sinapo = cut::herc_generic_apo(p0.p);
         for (int f = 0; f < n; ++f) {
             const cut::FramePlan& pl = s.slots[f]->plan;
             if (pl.p.sequence != HERCULES) CUT_FAIL("batch: graph mode runs HERCULES frames (frame " << f << ")");
-            if ((pl.p.focus_type == 4) != focus || (pl.p.apo_type == RX_TO_SIN) != sinapo)
-                CUT_FAIL("batch: the frames of a batch must share focus type and apodization (frame " << f << ")");
+            if ((pl.p.focus_type == 4) != focus || cut::herc_generic_apo(pl.p) != sinapo)
+                CUT_FAIL("batch: the frames of a batch must share focus type and apodization kernel (sine or exact "
+                         "Hann for volumes reaching z <= 0, or the Hann polynomial) (frame " << f << ")");
         }
         std::vector<cut::BatchFrameDev> fd(n);
         int n_items = 0;

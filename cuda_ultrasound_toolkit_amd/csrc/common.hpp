@@ -209,6 +209,9 @@ struct BatchFrameDev {
     int tiles, nxt, nyt, nzt;
 };
 void das_tile_grid(const DasParams& p, int& nxt, int& nyt, int& nzt);
+// HERCULES launches the generic-apodization kernels: the sine apodization, or
+// a Hann volume with a voxel at z <= 0 or a negative F-number (exact weight).
+bool herc_generic_apo(const DasParams& p);
 bool launch_batch_start(int* d_counters, int n, long long* d_start, hipStream_t st);
 bool launch_das_batch(const BatchFrameDev* d_frames, const int2* d_items, int n_items, int grid, int* d_next,
                       int* d_done, long long* d_finish, const DasOptions& o, bool focus, bool sinapo, int fmt,

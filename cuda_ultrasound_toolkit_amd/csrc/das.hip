@@ -155,6 +155,24 @@ __device__ __forceinline__ float tap_weight(float rho2, float kz2j, float apo_c)
     }
 }
 
+// The reference's F-number Hann weight for any sign of the depth and the
+// F-number: cos^2(pi min(a, 1/2)) with a = fn rho / (2 z)
+// (beamformer_utils.cuh:143-154), and 0 unless it passes the threshold
+// (apo > 0.1).  For a voxel above the array (z < 0) a is negative and
+// unbounded below, and the weight comes back above 0.1 periodically in |a|.
+// cos^2 has period 1 in a, so a is reduced to r = a - rint(a) in [-1/2, 1/2]
+// and the weight is the polynomial form in r^2 with its threshold test, the
+// same decision and value as the z > 0 path (a cosine's error would widen
+// the band of taps whose inclusion differs from the reference's).  fminf maps
+// a NaN ratio to 1/2 (weight 0), as the reference's does.  kzj = fn / (2 z).
+__device__ __forceinline__ float hann_weight_any(float rho2, float kzj)
+{
+    const float a = fminf(kzj * fsqrt(rho2), 0.5f);
+    const float r = a - rintf(a);
+    const float s = r * r;
+    return s < HANN_S_THR ? hann_poly(s) : 0.0f;
+}
+
 // Fractional sample index (txd + |rx| + fz) * spm + delay, clamped to [1, S-2].
 __device__ __forceinline__ float tap_index(float r2, float spm, float basej, float hi_clamp)
 {
@@ -246,6 +264,7 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
     // ~0.06 m, the same error for every tap of a plane (2e-5 samples on both
     // paths), which does not average out over the taps as rounding does.
     float z2[HT_Z], kz2[HT_Z], base[HT_Z];
+    float kzs[HT_Z];  // fn / (2 z), signed: the weight of volumes with a voxel at z <= 0
     double base_d[HT_Z];  // the same in FP64, anchored per channel block below
     bool zvalid[HT_Z];
 #pragma unroll
@@ -257,6 +276,7 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
         z2[j] = uni((float)(zd * zd));
         const float kz = p.fn_rx / z * 0.5f;
         kz2[j] = uni(kz * kz);
+        kzs[j] = kz;
         double txd;
         if constexpr (FOCUS) {
             const double ty = vy_d, tzz = zd - (double)fz;
@@ -309,6 +329,10 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
     }
 
     // Aperture radius (Hann): rho < 2*A*z/fn.  Unbounded when fn <= 0 or z <= 0.
+    // The generic instantiation (SINAPO) serves the sine apodization and, for
+    // Hann, volumes the polynomial form does not cover (a voxel at z <= 0 or a
+    // negative F-number: the launch routes them here): the exact weight per tap.
+    const bool hann_exact = SINAPO && p.apo_type != RX_TO_SIN;
     const bool aperture_bounded = !SINAPO && p.fn_rx > 0.0f && Z0 > 0.0f && p.pitch_x > 0.0f;
     const bool unbounded = !(p.fn_rx > 0.0f);  // no F-number: every lane is inside the aperture
     const float rmax_ap = aperture_bounded ? 2.0f * (HANN_A_THR * 1.001f) * Z1 / p.fn_rx + 1e-6f : 0.0f;
@@ -358,7 +382,7 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
             const size_t toff = (size_t)t * p.C * rowlen;
             if (neg) flip();
             float apo_y = 1.0f;
-            if constexpr (SINAPO) apo_y = sin_apo_to(ey, p.xdc_max_y, p.to_power);
+            if constexpr (SINAPO) if (!hann_exact) apo_y = sin_apo_to(ey, p.xdc_max_y, p.to_power);
 
             // Channels go in blocks of up to 64.  Wave 0 bounds the windows of a
             // block's channels, one per lane (the sample span any voxel of the
@@ -492,6 +516,7 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
                     rho2 = fmaf(dx, dx, dy2);
                     apo_c = 0.0f;
                     if constexpr (SINAPO) {
+                        if (hann_exact) return true;  // per tap, below
                         apo_c = sin_apo_to(ex, p.xdc_max_x, p.to_power) * apo_y;
                         return apo_c > 0.1f;  // uniform: depends on the element only
                     } else {
@@ -630,7 +655,8 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
                         for (int j = 0; j < HT_Z; ++j) {
                             float w;
                             if constexpr (SINAPO) {
-                                w = apo_c;
+                                w = hann_exact ? hann_weight_any(rho2, kzs[j]) : apo_c;
+                                if (CF && w == 0.0f) continue;
                             } else if constexpr (CF) {
                                 const float s = rho2 * kz2[j];
                                 if (!(s < HANN_S_THR)) continue;
@@ -651,7 +677,8 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
                     const size_t grow = toff + (size_t)(cb + cc) * rowlen;
 #pragma unroll
                     for (int j = 0; j < HT_Z; ++j) {
-                        const float w = tap_weight<SINAPO>(rho2, kz2[j], apo_c);
+                        const float w = SINAPO && hann_exact ? hann_weight_any(rho2, kzs[j])
+                                                             : tap_weight<SINAPO>(rho2, kz2[j], apo_c);
                         if (w == 0.0f) continue;
                         const float x = tap_index(rho2 + z2[j], p.spm, base[j], hi_clamp);
                         const float2 v = interp_global<CUBIC, IQK>(iq, grow, x, p.S, iq_inv);
@@ -1192,6 +1219,14 @@ int vox_waves(long long nvox)
 }
 }  // namespace
 
+bool herc_generic_apo(const DasParams& p)
+{
+    if (p.apo_type == RX_TO_SIN) return true;
+    // the depths as the kernel forms them (FP64 grid, rounded to float)
+    const float z0 = (float)((double)p.min_z), z1 = (float)((double)p.min_z + (double)(p.Nz - 1) * (double)p.res_z);
+    return !(p.fn_rx >= 0.0f) || !(fminf(z0, z1) > 0.0f);
+}
+
 size_t das_workspace_bytes(const DasParams& p)
 {
     if (p.sequence == RCA_TPW) return (size_t)MAX_ACQ_COUNT * sizeof(double2);
@@ -1222,7 +1257,7 @@ bool launch_herc_fmt(int fmt, dim3 grid, hipStream_t st, const void* iq, float i
         return true;
     }
     if constexpr (SA) {
-        CUT_FAIL("DAS: staged IQ formats support Hann apodization only");
+        CUT_FAIL("DAS: staged IQ formats support Hann apodization of volumes below the array (z > 0, F-number >= 0) only");
     } else {
         if (fmt == IQ_F16) hercules_kernel<CU, FO, CFV, SA, IQ_F16><<<grid, 256, 0, st>>>(iq, inv, p, hg, vol, part);
         else hercules_kernel<CU, FO, CFV, SA, IQ_BF16><<<grid, 256, 0, st>>>(iq, inv, p, hg, vol, part);
@@ -1265,7 +1300,7 @@ bool launch_batch_fmt(int fmt, int grid, hipStream_t st, const BatchFrameDev* fr
         return true;
     }
     if constexpr (SA) {
-        CUT_FAIL("DAS: staged IQ formats support Hann apodization only");
+        CUT_FAIL("DAS: staged IQ formats support Hann apodization of volumes below the array (z > 0, F-number >= 0) only");
     } else {
         if (fmt == IQ_F16)
             hercules_batch_kernel<CU, FO, CFV, SA, IQ_F16><<<grid, 256, 0, st>>>(fr, items, n, next, done, finish);
@@ -1345,7 +1380,7 @@ bool launch_das(const void* iq, int iq_fmt, float iq_scale, const DasParams& p, 
         const int G = p.readi_groups > 1 ? p.readi_groups : 1;
         const size_t nvox = (size_t)p.Nx * p.Ny * p.Nz;
         const bool focus = p.focus_type == 4;
-        const bool sinapo = p.apo_type == RX_TO_SIN;
+        const bool sinapo = herc_generic_apo(p);
         const bool cf = o.cf_hercules != 0;
         const int dflt = das_splits(tiles, G * p.T);
         auto fits = [&](int sp) {

@@ -470,10 +470,17 @@ unsigned long long das_active_taps(const DasParams& p)
                             const double dx = ex - ((double)p.min_x + ix * (double)p.res_x);
                             const double dy = ey - ((double)p.min_y + iy * (double)p.res_y);
                             const double rho = std::sqrt(dx * dx + dy * dy);
-                            if (p.fn_rx <= 0.0f) { n += p.Nz; continue; }
                             for (int iz = 0; iz < p.Nz; ++iz) {
                                 const double z = (double)p.min_z + iz * (double)p.res_z;
-                                if (z > 0 && p.fn_rx * rho / (2.0 * z) < A) n += 1;
+                                const double a = p.fn_rx * rho / (2.0 * z);
+                                if (z > 0 && p.fn_rx >= 0.0f) {
+                                    if (a < A) n += 1;  // cos^2 decreasing on [0, 1/2]
+                                } else {
+                                    // beamformer_utils.cuh:143-154 for any sign: cos^2(pi min(a, 1/2)),
+                                    // NaN (0/0) mapped to 1/2 as by fminf
+                                    const double c = std::cos(M_PI * std::fmin(a, 0.5));
+                                    if (c * c > 0.1) n += 1;
+                                }
                             }
                         }
                 }

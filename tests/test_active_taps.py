@@ -59,6 +59,23 @@ def _hercules_no_fnumber():
     return a
 
 
+def _hercules_above_array(fn):
+    """Depths from -3.125 to 8.875 mm: voxels above the array (z < 0), where
+    the reference's cos^2(pi min(fn rho / 2z, 1/2)) oscillates in |a| instead
+    of falling to zero, and F-number 0 or 1."""
+    def make():
+        a = configs.rca_small(seed=3, generate=False, T=8, n=14, z_range=(-3.125e-3, 8.875e-3))
+        a.bp.fn_rx = fn
+        return a
+    return make
+
+
+def _hercules_negative_fnumber():
+    a = configs.rca_small(seed=3, generate=False, T=8, n=12)
+    a.bp.fn_rx = -1.0
+    return a
+
+
 def _hercules_rca3d_subgrid():
     """The benchmark geometry (128 x 128 aperture, 5-60 mm) on a coarse grid."""
     a = configs.rca_3d(seed=2, generate=False, T=128, n=12, S=4096)
@@ -88,6 +105,9 @@ CASES = {
     "hercules_readi_g2_hadamard": _hercules_readi(2, L.HADAMARD),
     "hercules_readi_g4_walsh": _hercules_readi(4, L.WALSH),
     "hercules_no_fnumber": _hercules_no_fnumber,
+    "hercules_above_array_fn1": _hercules_above_array(1.0),
+    "hercules_above_array_fn0": _hercules_above_array(0.0),
+    "hercules_negative_fnumber": _hercules_negative_fnumber,
     "hercules_rca3d_geometry": _hercules_rca3d_subgrid,
     "forces_fn0": _forces(0.0),
     "forces_fn1": _forces(1.0),

@@ -253,6 +253,40 @@ def test_das_hercules_long_windows(torch_dev, orc, cubic, nz):
     _das_case(acq, orc, torch_dev, cubic, 1e-4)
 
 
+@pytest.mark.parametrize("cubic", [False, True])
+@pytest.mark.parametrize("fn", [1.0, 0.0])
+def test_das_hercules_above_array(torch_dev, orc, cubic, fn):
+    """Depths from -3.125 to 8.875 mm: above the array the reference's Hann
+    weight cos^2(pi min(fn rho / 2z, 1/2)) oscillates in |a| instead of
+    falling to zero; such volumes take the exact per-tap weight.
+
+    The weight jumps from 0.1 to 0 at the threshold, so a tap whose a lies
+    within FP32 rounding of it (~1e-7 relative) is included by one
+    arithmetic and not the other, moving its voxel by 0.1 x a sample.  This
+    grid keeps every tap at least 2e-6 (relative) from the threshold (FP64
+    scan of all taps; the -3..9 mm grid has taps 5e-9 from it, and one such
+    tap put a single voxel 1.3 % off)."""
+    print(f"das hercules above array fn={fn} cubic={cubic}", flush=True)
+    acq = configs.rca_small(seed=27, z_range=(-3.125e-3, 8.875e-3))
+    acq.bp.fn_rx = fn
+    _das_case(acq, orc, torch_dev, cubic, 1e-4)
+
+
+def test_das_hercules_above_array_cf_and_negative_fnumber(torch_dev, orc):
+    print("das hercules above array cf / negative F-number", flush=True)
+    # the grid of test_das_hercules_above_array (no tap within 8e-7 of the
+    # apodization threshold for F-number 1 or -1)
+    acq = configs.rca_small(seed=28, z_range=(-3.125e-3, 8.875e-3))
+    acq.bp.coherency_weighting = 0.5
+    _das_case(acq, orc, torch_dev, False, 1e-4, cf=True)
+    # a negative F-number below the array (3-14 mm, |a| < 2.5): near z = 0 the
+    # negative ratio a = fn rho / 2z is unbounded and cos^2(pi a) is beyond FP32
+    # (on the grid above, a depth of 5 um makes |a| ~ 1300)
+    acq = configs.rca_small(seed=29)
+    acq.bp.fn_rx = -1.0
+    _das_case(acq, orc, torch_dev, True, 1e-4)
+
+
 def test_das_hercules_sin_apodization(torch_dev, orc):
     print("das hercules RX_TO_SIN", flush=True)
     acq = configs.rca_small(seed=23)
