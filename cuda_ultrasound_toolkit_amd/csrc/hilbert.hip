@@ -28,6 +28,7 @@
 #include <hip/hip_fp16.h>
 
 #include <algorithm>
+#include <type_traits>
 
 namespace cut {
 
@@ -215,30 +216,49 @@ __device__ __forceinline__ float load_at(const void* base, unsigned stride, unsi
     else return ((const float*)base)[idx];
 }
 
+// Value at index idx of a pass's input; a loader that also takes the slot
+// m * R + r of the value in the thread's tile is a row held in registers.
+template <typename LoadF>
+__device__ __forceinline__ float2 pass_load(LoadF& load, int idx, int slot)
+{
+    if constexpr (std::is_invocable_v<LoadF&, int, int>) return load(idx, slot);
+    else return load(idx);
+}
+
+// Twiddle powers of the thread's butterflies in a pass: from registers when
+// the caller holds them (PRE: tpre, one per butterfly), else from the table.
+template <int S, int R, int VPT, bool PRE>
+__device__ __forceinline__ void pass_twiddles(int tid, int p, const float2* __restrict__ tw,
+                                              const TwPow<R>* tpre, TwPow<R>* tp)
+{
+    constexpr int P = S / VPT;
+    constexpr int PER = VPT / R;
+#pragma unroll
+    for (int m = 0; m < PER; ++m) {
+        const int i = tid + m * P;
+        if constexpr (PRE) tp[m] = tpre[m];
+        else tp[m] = tw_pow_load<R>(tw, (i & (p - 1)) * (S / (p * R)), S - 1);
+    }
+}
+
 // One Stockham pass over an S-point row held by P = S/VPT threads.  The
 // thread holds VPT values (the FFT tile: 16 or 8): VPT/R butterflies of radix
 // R.  Load phase -> (barrier) -> twiddle + DFT + store phase.
-template <int S, int R, bool INV, int VPT, typename LoadF, typename StoreF>
+template <int S, int R, bool INV, int VPT, bool PRE = false, typename LoadF, typename StoreF>
 __device__ __forceinline__ void stockham_pass(int tid, int p, const float2* __restrict__ tw, LoadF load,
-                                              StoreF store, bool sync_between)
+                                              StoreF store, bool sync_between, const TwPow<R>* tpre = nullptr)
 {
     constexpr int P = S / VPT;
     constexpr int NB = S / R;
     constexpr int PER = VPT / R;
     TwPow<R> tp[PER];
-    if (p > 1) {
-#pragma unroll
-        for (int m = 0; m < PER; ++m) {
-            const int i = tid + m * P;
-            tp[m] = tw_pow_load<R>(tw, (i & (p - 1)) * (S / (p * R)), S - 1);
-        }
-    }
+    if (p > 1) pass_twiddles<S, R, VPT, PRE>(tid, p, tw, tpre, tp);
     float2 u[VPT];
 #pragma unroll
     for (int m = 0; m < PER; ++m) {
         const int i = tid + m * P;
 #pragma unroll
-        for (int r = 0; r < R; ++r) u[m * R + r] = load(i + r * NB);
+        for (int r = 0; r < R; ++r) u[m * R + r] = pass_load(load, i + r * NB, m * R + r);
     }
     if (sync_between) __syncthreads();
 #pragma unroll
@@ -270,9 +290,13 @@ __device__ __forceinline__ uint32_t bf16_bits(float f)
 // pass reads through first_load (global memory, or LDS when first_lds), the
 // last writes through last_store; a barrier follows every pass whose output
 // later passes or transforms read.
-template <int LOGN, bool INV, int VPT = 16, typename LoadF, typename StoreF>
+// With PRE, tpre holds the thread's twiddle powers of the radix-VPT passes
+// after the first (one TwPow per pass: a radix-VPT pass is one butterfly per
+// thread).
+template <int LOGN, bool INV, int VPT = 16, bool PRE = false, typename LoadF, typename StoreF>
 __device__ __forceinline__ void fft_passes(int tid, const float2* __restrict__ tw, float2* lds, LoadF first_load,
-                                           bool first_lds, StoreF last_store, bool last_global)
+                                           bool first_lds, StoreF last_store, bool last_global,
+                                           const TwPow<VPT>* tpre = nullptr)
 {
     static_assert(VPT == 8 || VPT == 16, "FFT tile: 8 or 16 values per thread");
     constexpr int LV = VPT == 16 ? 4 : 3;
@@ -289,8 +313,9 @@ __device__ __forceinline__ void fft_passes(int tid, const float2* __restrict__ t
             if (last) stockham_pass<N, VPT, INV, VPT>(tid, p, tw, first_load, last_store, first_lds);
             else stockham_pass<N, VPT, INV, VPT>(tid, p, tw, first_load, store_lds, first_lds);
         } else {
-            if (last) stockham_pass<N, VPT, INV, VPT>(tid, p, tw, load_lds, last_store, true);
-            else stockham_pass<N, VPT, INV, VPT>(tid, p, tw, load_lds, store_lds, true);
+            const TwPow<VPT>* tq = PRE ? tpre + (q - 1) : nullptr;
+            if (last) stockham_pass<N, VPT, INV, VPT, PRE>(tid, p, tw, load_lds, last_store, true, tq);
+            else stockham_pass<N, VPT, INV, VPT, PRE>(tid, p, tw, load_lds, store_lds, true, tq);
         }
         if (!(last && last_global)) __syncthreads();
         p *= VPT;
@@ -305,21 +330,15 @@ __device__ __forceinline__ void fft_passes(int tid, const float2* __restrict__ t
 // Two transforms of the same length and direction in lockstep: one set of
 // passes, barriers and twiddle loads for both, two independent butterfly
 // chains per thread.
-template <int S, int R, bool INV, int VPT, typename LA, typename LB, typename SA, typename SB>
+template <int S, int R, bool INV, int VPT, bool PRE = false, typename LA, typename LB, typename SA, typename SB>
 __device__ __forceinline__ void stockham_pass2(int tid, int p, const float2* __restrict__ tw, LA la, LB lb, SA sa,
-                                               SB sb, bool sync_between)
+                                               SB sb, bool sync_between, const TwPow<R>* tpre = nullptr)
 {
     constexpr int P = S / VPT;
     constexpr int NB = S / R;
     constexpr int PER = VPT / R;
     TwPow<R> tp[PER];
-    if (p > 1) {
-#pragma unroll
-        for (int m = 0; m < PER; ++m) {
-            const int i = tid + m * P;
-            tp[m] = tw_pow_load<R>(tw, (i & (p - 1)) * (S / (p * R)), S - 1);
-        }
-    }
+    if (p > 1) pass_twiddles<S, R, VPT, PRE>(tid, p, tw, tpre, tp);
     float2 ua[VPT], ub[VPT];
 #pragma unroll
     for (int m = 0; m < PER; ++m) {
@@ -356,10 +375,10 @@ __device__ __forceinline__ void stockham_pass2(int tid, int p, const float2* __r
 }
 
 // fft_passes for two transforms in lockstep (rows in lds_a and lds_b).
-template <int LOGN, bool INV, int VPT, typename LA, typename LB, typename SA, typename SB>
+template <int LOGN, bool INV, int VPT, bool PRE = false, typename LA, typename LB, typename SA, typename SB>
 __device__ __forceinline__ void fft_passes2(int tid, const float2* __restrict__ tw, float2* lds_a, float2* lds_b,
                                             LA first_a, LB first_b, bool first_lds, SA last_a, SB last_b,
-                                            bool last_global)
+                                            bool last_global, const TwPow<VPT>* tpre = nullptr)
 {
     static_assert(VPT == 8 || VPT == 16, "FFT tile: 8 or 16 values per thread");
     constexpr int LV = VPT == 16 ? 4 : 3;
@@ -378,8 +397,9 @@ __device__ __forceinline__ void fft_passes2(int tid, const float2* __restrict__ 
             if (last) stockham_pass2<N, VPT, INV, VPT>(tid, p, tw, first_a, first_b, last_a, last_b, first_lds);
             else stockham_pass2<N, VPT, INV, VPT>(tid, p, tw, first_a, first_b, sa, sb, first_lds);
         } else {
-            if (last) stockham_pass2<N, VPT, INV, VPT>(tid, p, tw, la, lb, last_a, last_b, true);
-            else stockham_pass2<N, VPT, INV, VPT>(tid, p, tw, la, lb, sa, sb, true);
+            const TwPow<VPT>* tq = PRE ? tpre + (q - 1) : nullptr;
+            if (last) stockham_pass2<N, VPT, INV, VPT, PRE>(tid, p, tw, la, lb, last_a, last_b, true, tq);
+            else stockham_pass2<N, VPT, INV, VPT, PRE>(tid, p, tw, la, lb, sa, sb, true, tq);
         }
         if (!(last && last_global)) __syncthreads();
         p *= VPT;
@@ -500,6 +520,215 @@ __global__ __launch_bounds__(fft_block(LOGS, VPT)) void hilbert_pair_kernel(Hilb
         __asm__ volatile("" : "+s"(tw_b));
         fft_passes<LOGS, true, VPT>(tid, tw_b, lds, load_yb, false, store_b, true);
     }
+}
+
+// hilbert_pair_kernel for 4096-point rows and 16 values per thread as a fixed
+// grid of workgroups, each walking row pairs blockIdx.x, + gridDim.x, ...
+// Each iteration requests the next pair's input rows once the first forward
+// pass has consumed the current pair's (into the same registers); the next
+// iteration's first pass consumes them.  Vector-memory loads retire in order,
+// so a load issued after the prefetch would make the wave wait for the
+// prefetch as well: the twiddle powers of the two twiddled passes (p = 16,
+// 256; the inverse uses their conjugates) depend on the thread alone and are
+// loaded once and held in registers, and the filter values of the split are
+// requested (L2 hits) just before the prefetch.  Arithmetic, LDS
+// layout and rounding are those of hilbert_pair_kernel<12, KIND, FILTER, 16>.
+template <int KIND>
+using HilRaw = std::conditional_t<KIND == 1, int, float>;
+
+template <int KIND>
+__device__ __forceinline__ HilRaw<KIND> load_raw(const void* base, unsigned stride, unsigned idx)
+{
+    if constexpr (KIND == 0) return ((const float*)base)[idx * stride];
+    else if constexpr (KIND == 1) return (int)((const int16_t*)base)[idx];
+    else return ((const float*)base)[idx];
+}
+
+constexpr int HIL_STREAM_LOGS = 12;
+// Word 3 of a raw (untyped, stride 0) buffer resource on gfx9-family GPUs:
+// 32-bit data format, so that the range check compares byte offsets with
+// the resource's size and drops stores at or beyond it.
+constexpr int HIL_RSRC_WORD3 = 0x00020000;
+
+template <int KIND, bool FILTER, int OFMT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+void hilbert_pair_stream_kernel(HilbertDev in, int rows, const float2* __restrict__ tw,
+                                const float2* __restrict__ F, void* __restrict__ out, float oscale)
+{
+    constexpr int LOGS = HIL_STREAM_LOGS;
+    constexpr int S = 1 << LOGS;
+    constexpr int H = S / 2;
+    constexpr int VPT = 16;
+    constexpr int P = S / VPT;
+    constexpr int NB = S / VPT;
+    constexpr int PADDED = S + S / 16;
+    static_assert(P == 256, "one row pair per 256-thread workgroup");
+    __shared__ float2 buf[PADDED];
+    __shared__ float2 buf2[PADDED];
+    float2* const lds = buf;
+    float2* const lds2 = buf2;
+
+    const int tid0 = threadIdx.x;
+    const int npairs = (rows + 1) / 2;
+    TwPow<VPT> tpre[LOGS / 4 - 1];
+#pragma unroll
+    for (int q = 1, p = VPT; q < LOGS / 4; ++q, p *= VPT)
+        tpre[q - 1] = tw_pow_load<VPT>(tw, (tid0 & (p - 1)) * (S / (p * VPT)), S - 1);
+    // (the scheduler keeps the order of these loads and the rows' below)
+    __builtin_amdgcn_sched_barrier(0);
+
+    const unsigned istride = (unsigned)in.stride;
+    using Raw = HilRaw<KIND>;
+    // Loads of a pair's two rows, unconditional so that the wait for them
+    // has one count on every path: past the last pair the last pair is
+    // fetched again (and not used); a lone last row's partner is read from
+    // the row itself and zeroed where it is consumed.
+    auto fetch = [&](int pair, Raw* xa, Raw* xb, int tid) {
+        const int pc = pair < npairs ? pair : npairs - 1;
+        const int ra = 2 * pc, rb = ra + 1 < rows ? ra + 1 : ra;
+        const void* base_a = row_base<KIND>(in, ra, S);
+        const void* base_b = row_base<KIND>(in, rb, S);
+#pragma unroll
+        for (int r = 0; r < VPT; ++r) xa[r] = load_raw<KIND>(base_a, istride, (unsigned)(tid + r * NB));
+#pragma unroll
+        for (int r = 0; r < VPT; ++r) xb[r] = load_raw<KIND>(base_b, istride, (unsigned)(tid + r * NB));
+    };
+    // The walk is rotated: an iteration runs its pair's transforms from the
+    // second forward pass on, then the next pair's first pass, which consumes
+    // the input rows (ca, cb) requested one iteration earlier, and requests
+    // the following pair's rows into the same registers, after the filter
+    // values of the next split (older loads: waiting for them leaves the
+    // rows in flight).  The wait for the rows then follows this pair's
+    // output stores on every path, so they stay in flight too.
+    Raw ca[VPT], cb[VPT];
+    float2 fk[H / P];
+    float2 fH = make_float2(1.0f, 0.0f);
+    auto store_lds = [&](int idx, float2 v) { lds[lds_idx(idx)] = v; };
+    auto load_lds = [&](int idx) { return lds[lds_idx(idx)]; };
+    auto first_and_prefetch = [&](int pair, int tid) {
+        const bool act_b = 2 * pair + 1 < rows;
+        auto first = [&](int, int slot) { return make_float2((float)ca[slot], act_b ? (float)cb[slot] : 0.0f); };
+        stockham_pass<S, VPT, false, VPT>(tid, 1, tw, first, store_lds, false);
+        if constexpr (FILTER) {
+#pragma unroll
+            for (int j = 0; j < H / P; ++j) fk[j] = F[tid + j * P];
+            fH = F[H];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        fetch(pair + gridDim.x, ca, cb, tid);
+        __syncthreads();
+    };
+    fetch(blockIdx.x, ca, cb, tid0);
+    first_and_prefetch(blockIdx.x, tid0);
+
+    const float half_w = 0.75f / (float)S;
+    constexpr unsigned ESZ = OFMT == IQ_F32 ? sizeof(float2) : sizeof(uint32_t);
+    // the grid holds at most one workgroup per pair: every workgroup has a
+    // first pair (a loop without an entry test, past which the loads above
+    // would be sunk)
+    int pair = blockIdx.x;
+    do {
+        // the pass addresses derived from the thread index are formed anew in
+        // each iteration rather than hoisted out of the walk and held (their
+        // registers spilled)
+        int tid = tid0;
+        __asm__ volatile("" : "+v"(tid));
+        tid &= P - 1;
+
+#pragma unroll
+        for (int q = 1, p = VPT; q < LOGS / 4; ++q, p *= VPT) {
+            stockham_pass<S, VPT, false, VPT, true>(tid, p, tw, load_lds, store_lds, true, tpre + (q - 1));
+            __syncthreads();
+        }
+
+        auto split = [&](int k, float2 f) {
+            const float2 zk = lds[lds_idx(k)];
+            const float2 zn = lds[lds_idx((S - k) & (S - 1))];
+            const float w = k == 0 ? 0.5f * half_w : half_w;
+            float2 ya = make_float2((zk.x + zn.x) * w, (zk.y - zn.y) * w);
+            float2 yb = make_float2((zk.y + zn.y) * w, (zn.x - zk.x) * w);
+            if constexpr (FILTER) {
+                ya = cmul(ya, f);
+                yb = cmul(yb, f);
+            }
+            lds[lds_idx(k)] = ya;
+            lds2[lds_idx(k)] = yb;
+        };
+#pragma unroll
+        for (int j = 0; j < H / P; ++j) split(tid + j * P, fk[j]);
+        if (tid == 0) split(H, fH);
+        __syncthreads();
+
+        // Rows a and b are adjacent in the output: one buffer resource spans
+        // them, sized to row a alone when b is the lone last row's zero
+        // partner, so that b's stores are dropped by the range check rather
+        // than skipped by a branch.
+        const int ra = 2 * pair;
+        const bool act_b = ra + 1 < rows;
+        char* const out_a = static_cast<char*>(out) + (size_t)ra * S * ESZ;
+        const __amdgpu_buffer_rsrc_t orsrc =
+            __builtin_amdgcn_make_buffer_rsrc(out_a, 0, (act_b ? 2 : 1) * S * ESZ, HIL_RSRC_WORD3);
+        auto store_out = [&](unsigned byte_off, float2 v) {
+            if constexpr (OFMT == IQ_F32) {
+                __builtin_amdgcn_raw_buffer_store_b64(
+                    __builtin_bit_cast(HIP_vector_base<unsigned, 2>::Native_vec_, v), orsrc, byte_off, 0, 0);
+            } else if constexpr (OFMT == IQ_F16) {
+                const __half2 h = __floats2half2_rn(v.x * oscale, v.y * oscale);
+                __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, h), orsrc, byte_off, 0, 0);
+            } else {
+                __builtin_amdgcn_raw_buffer_store_b32(bf16_bits(v.x) | (bf16_bits(v.y) << 16), orsrc, byte_off, 0,
+                                                      0);
+            }
+        };
+        auto load_ya = [&](int idx) { return idx > H ? make_float2(0.0f, 0.0f) : lds[lds_idx(idx)]; };
+        auto load_yb = [&](int idx) { return idx > H ? make_float2(0.0f, 0.0f) : lds2[lds_idx(idx)]; };
+        auto store_a = [&](int idx, float2 v) { store_out((unsigned)idx * ESZ, v); };
+        auto store_b = [&](int idx, float2 v) { store_out((unsigned)(S + idx) * ESZ, v); };
+        fft_passes2<LOGS, true, VPT, true>(tid, tw, lds, lds2, load_ya, load_yb, true, store_a, store_b, true, tpre);
+        __syncthreads();  // the next pair's first pass writes the row buffer this pair's last pass read
+        pair += gridDim.x;
+        if (pair < npairs) first_and_prefetch(pair, tid);
+    } while (pair < npairs);
+}
+
+// Workgroups of hilbert_pair_stream_kernel resident on the device at once
+// (per device, from the occupancy of the instantiation launched).
+template <int KIND, bool FILTER, int OFMT>
+static bool hil_stream_grid(int pairs, int& grid)
+{
+    static int wgs[64] = {};
+    int dev = 0;
+    CUT_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) CUT_FAIL("Hilbert: device ordinal " << dev << " out of range");
+    if (wgs[dev] == 0) {
+        int cus = 0, per_cu = 0;
+        CUT_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        CUT_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, reinterpret_cast<const void*>(&hilbert_pair_stream_kernel<KIND, FILTER, OFMT>), 256, 0));
+        wgs[dev] = std::max(1, cus * std::max(1, per_cu));
+    }
+    grid = std::min(pairs, wgs[dev]);
+    return true;
+}
+
+template <int KIND, bool FILTER, int OFMT>
+static bool launch_hilbert_stream_f(const HilbertDev& d, int rows, const float2* tw, const float2* F, void* out,
+                                    float oscale, hipStream_t st)
+{
+    int grid = 0;
+    if (!hil_stream_grid<KIND, FILTER, OFMT>((rows + 1) / 2, grid)) return false;
+    hilbert_pair_stream_kernel<KIND, FILTER, OFMT><<<grid, 256, 0, st>>>(d, rows, tw, F, out, oscale);
+    CUT_HIP_CHECK(hipGetLastError());
+    return true;
+}
+
+template <int KIND, bool FILTER>
+static bool launch_hilbert_stream(const HilbertDev& d, int rows, const float2* tw, const float2* F, void* out,
+                                  int ofmt, float oscale, hipStream_t st)
+{
+    if (ofmt == IQ_F32) return launch_hilbert_stream_f<KIND, FILTER, IQ_F32>(d, rows, tw, F, out, oscale, st);
+    if (ofmt == IQ_F16) return launch_hilbert_stream_f<KIND, FILTER, IQ_F16>(d, rows, tw, F, out, oscale, st);
+    return launch_hilbert_stream_f<KIND, FILTER, IQ_BF16>(d, rows, tw, F, out, oscale, st);
 }
 
 // Hilbert transform of rows of any length S <= 4096 that is not a power of
@@ -810,6 +1039,16 @@ static bool launch_hilbert_s(const HilbertInput& in, int rows, const float2* tw,
     constexpr int P = S / VPT;
     constexpr int RPW = P >= 256 ? 1 : 256 / P;     // row pairs per workgroup
     HilbertDev d{in.ptr, in.stride, in.map, in.R0, in.C};
+    if constexpr (LOGS == HIL_STREAM_LOGS && VPT == 16) {
+        if (F) {
+            if (in.kind == 0) return launch_hilbert_stream<0, true>(d, rows, tw, F, out, ofmt, oscale, st);
+            if (in.kind == 1) return launch_hilbert_stream<1, true>(d, rows, tw, F, out, ofmt, oscale, st);
+            return launch_hilbert_stream<2, true>(d, rows, tw, F, out, ofmt, oscale, st);
+        }
+        if (in.kind == 0) return launch_hilbert_stream<0, false>(d, rows, tw, F, out, ofmt, oscale, st);
+        if (in.kind == 1) return launch_hilbert_stream<1, false>(d, rows, tw, F, out, ofmt, oscale, st);
+        return launch_hilbert_stream<2, false>(d, rows, tw, F, out, ofmt, oscale, st);
+    }
     const int pairs = (rows + 1) / 2;
     dim3 grid((pairs + RPW - 1) / RPW);
     dim3 block(P * RPW);

@@ -183,6 +183,76 @@ def test_hilbert_odd_row_count(torch_dev, orc, S, taps):
     assert worst < 1e-4
 
 
+@pytest.mark.parametrize("taps", [0, 64])
+def test_hilbert_walk_many_pairs(torch_dev, orc, taps):
+    """4096-point rows take a fixed grid of workgroups (two per CU) that walk
+    the row pairs, each requesting the next pair's rows while it transforms
+    the current one.  1517 rows = 759 pairs: workgroups walk one or two pairs
+    and the last pair holds a lone row.  FP32 against the oracle, strided
+    input bit-equal to contiguous, FP16 / BF16 output against the rounding of
+    the FP32 output."""
+    import torch
+    S, T, Cc = 4096, 37, 41
+    print(f"hilbert walk S={S} rows={T * Cc} taps={taps}", flush=True)
+    assert api.rf_init((T * S, Cc), (S, Cc, T), L.NONE)
+    rng = np.random.default_rng(41 + taps)
+    x = (rng.standard_normal((T * Cc, S)) * np.geomspace(1.0, 300.0, T * Cc)[:, None]).astype(np.float32)
+    h = rng.standard_normal(taps).astype(np.float32) if taps else None
+    assert api.rf_set_match_filter(h)
+    try:
+        d_x = _t(x, torch_dev)
+        out = torch.full((T * Cc * S * 2,), np.nan, dtype=torch.float32, device=torch_dev)
+        assert api.rf_hilbert(d_x, 1, out)
+        torch.cuda.synchronize()
+        got = out.cpu().numpy().view(np.complex64).reshape(T * Cc, S)
+        want = orc.hilbert(x.astype(np.float64), None if h is None else h.astype(np.float64))
+        assert np.all(np.isfinite(got))
+        err = rel_l2(got, want)
+        worst = max(rel_l2(got[r], want[r]) for r in range(T * Cc))
+        print(f"  rel L2 {err:.2e}, worst row {worst:.2e}")
+        assert err < 1e-5
+        assert worst < 1e-4
+        xc = np.zeros((T * Cc, S), np.complex64)
+        xc.real = x
+        xc.imag = -7.0
+        out2 = torch.zeros_like(out)
+        assert api.rf_hilbert(_t(xc.view(np.float32), torch_dev), 2, out2)
+        torch.cuda.synchronize()
+        assert np.array_equal(out2.cpu().numpy(), out.cpu().numpy())
+        g32 = out.cpu().numpy().reshape(-1, 2)
+        for fmt in (1, 2):
+            words = torch.zeros(T * Cc * S, dtype=torch.int32, device=torch_dev)
+            ok, scale = api.rf_hilbert_format(d_x, 1, words, fmt)
+            assert ok
+            torch.cuda.synchronize()
+            w = words.cpu().numpy().view(np.uint32)
+            if fmt == 1:
+                v = (g32 * np.float32(scale)).astype(np.float16).view(np.uint16).astype(np.uint32)
+            else:
+                u = g32.view(np.uint32)
+                v = (u + np.uint32(0x7FFF) + ((u >> 16) & np.uint32(1))) >> 16  # nearest-even BF16
+            want_w = v[:, 0] | (v[:, 1] << 16)
+            same = float(np.mean(w == want_w))
+            print(f"  format {fmt}: {same:.6f} of the words equal the FP32 output's rounding")
+            assert same > 0.999
+    finally:
+        assert api.rf_set_match_filter(None)
+
+
+def test_beamform_decode_none_many_rows(lib, orc):
+    """decode = NONE at 4096 samples: the fused convert + Hilbert reads the
+    int16 rows through the channel map while walking several row pairs per
+    workgroup; 47 x 47 = 2209 rows (odd)."""
+    print("beamform decode none, 47 x 47 rows of 4096", flush=True)
+    acq = configs.rca_small(seed=13, T=47, S=4096, encoding=L.NONE)
+    acq.bp.decode = L.NONE
+    got = api.beamform_volume(acq.raw, acq.bp)
+    assert got is not None
+    err = rel_l2(got, orc.pipeline(acq))
+    print(f"  rel L2 {err:.2e}")
+    assert err < 1e-4
+
+
 def test_beamform_decode_none_odd_rows(lib, orc):
     """decode = NONE takes the fused convert + Hilbert kernel; 5 x 5 rows."""
     print("beamform decode none, 25 rows", flush=True)
