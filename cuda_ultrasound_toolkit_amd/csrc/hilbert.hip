@@ -17,7 +17,10 @@
 // int16/f32 through the channel map, so convert+Hilbert is one kernel when no
 // decode is needed), the spectral weight and filter are applied while the
 // two spectra are separated, and the last inverse pass writes each complex
-// row straight to HBM.  Other lengths are a cyclic convolution with the
+// row straight to HBM.  4096-point rows (16 values per thread) run as a
+// fixed grid of workgroups walking the row pairs, each requesting the next
+// pair's rows while it transforms the current one (hilbert_pair_stream_kernel).
+// Other lengths are a cyclic convolution with the
 // operator's kernel, evaluated by power-of-two FFTs of the zero-padded row
 // (hilbert_conv_kernel); rows longer than 4096 take HBM-staged four-step
 // transforms.  LDS rows are padded by one complex per 16 so the radix-16

@@ -239,13 +239,46 @@ def test_hilbert_walk_many_pairs(torch_dev, orc, taps):
         assert api.rf_set_match_filter(None)
 
 
-def test_beamform_decode_none_many_rows(lib, orc):
+@pytest.mark.parametrize("rows", [1, 2, 3, 1026])
+def test_hilbert_walk_row_counts(torch_dev, orc, rows):
+    """4096-point rows at the edges of the row-pair walk: one row (a lone row
+    with its zero partner), two, three, and 513 pairs (one more than a
+    512-workgroup grid holds, so the first workgroup walks two pairs while
+    the rest take one); 64-tap filter, FP32 output against the oracle."""
+    import torch
+    S = 4096
+    print(f"hilbert walk rows={rows}", flush=True)
+    assert api.rf_init((rows * S, 1), (S, 1, rows), L.NONE)
+    rng = np.random.default_rng(rows + 7)
+    x = (rng.standard_normal((rows, S)) * 100).astype(np.float32)
+    h = rng.standard_normal(64).astype(np.float32)
+    assert api.rf_set_match_filter(h)
+    try:
+        out = torch.full((rows * S * 2 + 64,), np.nan, dtype=torch.float32, device=torch_dev)
+        assert api.rf_hilbert(_t(x, torch_dev), 1, out)
+        torch.cuda.synchronize()
+        o = out.cpu().numpy()
+        assert np.all(np.isnan(o[rows * S * 2:]))  # nothing written past the last row
+        got = o[: rows * S * 2].view(np.complex64).reshape(rows, S)
+        want = orc.hilbert(x.astype(np.float64), h.astype(np.float64))
+        assert np.all(np.isfinite(got))
+        assert rel_l2(got, want) < 1e-5
+        assert max(rel_l2(got[r], want[r]) for r in range(rows)) < 1e-4
+    finally:
+        assert api.rf_set_match_filter(None)
+
+
+@pytest.mark.parametrize("dtype", [L.TYPE_I16, L.TYPE_F32])
+def test_beamform_decode_none_many_rows(lib, orc, dtype):
     """decode = NONE at 4096 samples: the fused convert + Hilbert reads the
-    int16 rows through the channel map while walking several row pairs per
-    workgroup; 47 x 47 = 2209 rows (odd)."""
-    print("beamform decode none, 47 x 47 rows of 4096", flush=True)
+    int16 (or float32) rows through the channel map while walking several row
+    pairs per workgroup; 47 x 47 = 2209 rows (odd)."""
+    print(f"beamform decode none, 47 x 47 rows of 4096, data type {dtype}", flush=True)
     acq = configs.rca_small(seed=13, T=47, S=4096, encoding=L.NONE)
     acq.bp.decode = L.NONE
+    if dtype == L.TYPE_F32:
+        acq.raw = (acq.raw.astype(np.float32) * np.float32(0.37)).astype(np.float32)
+        acq.bp.data_type = L.TYPE_F32
     got = api.beamform_volume(acq.raw, acq.bp)
     assert got is not None
     err = rel_l2(got, orc.pipeline(acq))
