@@ -1192,10 +1192,10 @@ __global__ __launch_bounds__(1024) void tpw_kernel(const float2* __restrict__ iq
 }
 
 // Launch tuning (tuning.hpp): the transmit split of the single-frame HERCULES
-// launch (the heuristic das_splits, or the fastest of 1, 2, 4, 8 and 16
-// splits) and the waves per voxel group of the per-voxel kernels (vox_waves,
-// or the fastest of 1 to 16), measured the first time a shape is seen
-// (outside stream capture).
+// launch (the heuristic das_splits, or, with timed tuning on, the fastest of
+// 1, 2, 4, 8 and 16 splits) and the waves per voxel group of the per-voxel
+// kernels (vox_waves, or the fastest of 1 to 16), measured the first time a
+// shape is seen (outside stream capture).
 namespace {
 constexpr int TUNE_MAX_SPLITS = 16;
 
@@ -1240,10 +1240,13 @@ size_t das_workspace_bytes(const DasParams& p)
 
 int das_splits(long long tiles, int gt_total)
 {
-    // aim for >= 4096 workgroups (4 resident per CU x 256 CUs x 4 rounds), at
-    // least 16 transmits per split
-    long long s = (4096 + tiles - 1) / tiles;
-    s = std::min<long long>(s, std::max(1, gt_total / 16));
+    // aim for >= 16384 workgroups (4 resident per CU x 256 CUs x 16 rounds:
+    // the deepest tiles carry ~z^2 more taps, so many small items balance the
+    // chip), at least 8 transmits per split.  On the 128^3 frame (1024 tiles,
+    // 128 transmits) this is 16, the timed tuner's choice too (16 / 8 / 4 / 2 /
+    // 1 splits: 7.39 / 8.2 / 9.3 / 10.2 / 10.5 ms).
+    long long s = (16384 + tiles - 1) / tiles;
+    s = std::min<long long>(s, std::max(1, gt_total / 8));
     return (int)std::max<long long>(1, std::min<long long>(s, 16));
 }
 

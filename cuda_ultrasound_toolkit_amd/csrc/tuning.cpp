@@ -3,6 +3,7 @@
 #include "tuning.hpp"
 
 #include <cstdlib>
+#include <cstring>
 #include <fstream>
 #include <map>
 #include <mutex>
@@ -14,10 +15,21 @@
 namespace cut {
 namespace {
 
+// Timed tuning of the choices that change the rounding (DAS split, waves per
+// voxel group, FFT tile) is off unless CUT_AUTOTUNE is set to a non-zero
+// value or cut_set_autotune(1) is called: by default every process takes the
+// same heuristic choices (or those of a loaded tuning file) and so computes
+// the same bits.
+bool autotune_from_env()
+{
+    const char* v = std::getenv("CUT_AUTOTUNE");
+    return v && *v && std::strcmp(v, "0") != 0;
+}
+
 struct TuneState {
     std::mutex mu;
     std::map<std::pair<int, TuneKey>, int> table;
-    bool autotune = true;
+    bool autotune = autotune_from_env();
     bool file_read = false;
 };
 

@@ -72,12 +72,16 @@ typedef struct CutFrame {
  * state and buffers across calls.  frame_ms (may be NULL): per frame, the time
  * from the start of the batch on the GPU to that frame's completion. */
 CUT_EXT_EXPORT bool cut_beamform_batch(const CutFrame* frames, int n, int mode, double* frame_ms);
-/* Autotuning, on by default: the batch DAS launch shape (grid size, work
- * order) per batch shape, the single-frame HERCULES transmit split (1, 2, 4,
- * 8 or 16 workgroups per tile, partial sums added in split order) and, for
- * FORCES, UFORCES and RCA_TPW, the waves per 64-voxel workgroup (1 to 16,
- * each taking a share of the voxel's outer loop) per frame shape; each is
- * timed on first use (that call synchronises) and the fastest kept. */
+/* Timed autotuning.  The batch DAS launch shape (grid size, work order) is
+ * tuned per batch shape by default: its candidates compute the same bits.
+ * The choices that change the rounding (~1e-7 relative) -- the single-frame
+ * HERCULES transmit split (1, 2, 4, 8 or 16 workgroups per tile, partial sums
+ * added in split order), for FORCES, UFORCES and RCA_TPW the waves per 64-voxel
+ * workgroup (1 to 16, each taking a share of the voxel's outer loop), and the
+ * FFT tile -- come from fixed heuristics by default, so every process
+ * computes the same bits, as the reference does.  on = 1 (or the environment
+ * variable CUT_AUTOTUNE=1) also times those on first use per shape (that call
+ * synchronises) and keeps the fastest; on = 0 turns off the batch tuning too. */
 CUT_EXT_EXPORT bool cut_set_autotune(int on);
 /* FFT tile of the Hilbert transforms (rows of up to 4096 samples): values
  * per thread in the LDS passes, 8 or 16; 0 (default) = tuned per row length,
@@ -86,7 +90,7 @@ CUT_EXT_EXPORT bool cut_set_autotune(int on);
 CUT_EXT_EXPORT bool cut_set_fft_tile(int values_per_thread);
 /* The tuned launch shapes (DAS split, waves per voxel group, FFT tile) differ
  * in the rounding of the results (~1e-7 relative), so two processes that tune
- * independently can differ in the last bits.  cut_tuning_save writes this
+ * independently (timed tuning on) can differ in the last bits.  cut_tuning_save writes this
  * process's choices to a text file, cut_tuning_load reads such a file (its
  * choices replace tuning for those shapes); with the environment variable
  * CUT_TUNING_FILE set, the library reads that file on first use and rewrites

@@ -1,8 +1,10 @@
 """Tuned launch shapes and reproducibility across processes.
 
-Two fresh processes that share a tuning file (CUT_TUNING_FILE: the first one
-tunes and writes it, the second reads it) beamform the same frame to the same
-bits, through every tuned choice (HERCULES transmit split, FFT tile).
+By default the choices that change the rounding (HERCULES transmit split, FFT
+tile) are fixed heuristics, so two fresh processes beamform the same frames to
+the same bits.  With timed tuning on (CUT_AUTOTUNE=1), two fresh processes that
+share a tuning file (CUT_TUNING_FILE: the first one tunes and writes it, the
+second reads it) do too, through every tuned choice.
 """
 import json
 import os
@@ -33,10 +35,28 @@ np.save(sys.argv[2], np.stack(outs))
 """
 
 
+def _clean_env(**kw):
+    env = {k: v for k, v in os.environ.items() if k not in ("CUT_TUNING_FILE", "CUT_AUTOTUNE")}
+    env.update(kw)
+    return env
+
+
+def test_default_processes_compute_the_same_bits(tmp_path):
+    print("two processes, default settings", flush=True)
+    vols = []
+    for i in range(2):
+        out = tmp_path / f"vol{i}.npy"
+        r = subprocess.run([sys.executable, "-c", CHILD, str(ROOT), str(out)], env=_clean_env(), capture_output=True,
+                           text=True, timeout=300)
+        assert r.returncode == 0, r.stderr[-3000:]
+        vols.append(np.load(out))
+    assert np.array_equal(vols[0], vols[1])
+
+
 def test_processes_sharing_a_tuning_file_compute_the_same_bits(tmp_path):
-    print("two processes, one tuning file", flush=True)
+    print("two processes, timed tuning, one tuning file", flush=True)
     tune = tmp_path / "tuning.txt"
-    env = dict(os.environ, CUT_TUNING_FILE=str(tune))
+    env = _clean_env(CUT_TUNING_FILE=str(tune), CUT_AUTOTUNE="1")
     vols = []
     for i in range(2):
         out = tmp_path / f"vol{i}.npy"
