@@ -336,6 +336,42 @@ def test_das_hercules_coherence_factor_extension(torch_dev, orc):
     _das_case(acq, orc, torch_dev, True, 1e-4, cf=True)
 
 
+@pytest.mark.parametrize("cubic", [False, True])
+def test_das_hercules_every_transmit_split(torch_dev, orc, tmp_path, cubic):
+    """Every transmit split of the HERCULES launch (1, 2, 4, 8 and 16 workgroups
+    per tile, forced through the tuning table for this shape) against the
+    oracle, and all within 1e-6 of one another (only the order in which the
+    partial sums are added differs)."""
+    import torch
+    print(f"das hercules every split cubic={cubic}", flush=True)
+    acq = configs.rca_small(seed=29, T=128, n=16)
+    bp = acq.bp
+    Nx, Ny, Nz = bp.output_points[0], bp.output_points[1], bp.output_points[2]
+    S, Cc, T = bp.dec_data_dim[0], bp.dec_data_dim[1], bp.dec_data_dim[2]
+    iq = orc.pipeline(acq, cubic=cubic, return_stages=True)["iq"].astype(np.complex64)
+    want, _ = orc.das(iq.astype(np.complex128), bp, cubic=cubic)
+    assert api.set_das_options(cubic=cubic, cf_hercules=False)
+    d_iq = _t(iq.view(np.float32), torch_dev)
+    vols = {}
+    try:
+        for sp in (1, 2, 4, 8, 16):
+            path = tmp_path / f"split{sp}.txt"
+            # table 0 (HERCULES split), key Nx Ny Nz T C S G fmt cubic cf focus generic
+            path.write_text(f"0 {Nx} {Ny} {Nz} {T} {Cc} {S} 1 0 {int(cubic)} 0 0 0 {sp}\n")
+            assert api.tuning_load(path)
+            vol = torch.zeros(want.size * 2, dtype=torch.float32, device=torch_dev)
+            assert api.das_device(d_iq, vol, bp)
+            torch.cuda.synchronize()
+            vols[sp] = vol.cpu().numpy().view(np.complex64).reshape(want.shape)
+            err = rel_l2(vols[sp], want)
+            print(f"  split {sp}: rel L2 {err:.3e}")
+            assert err < 1e-4
+    finally:
+        assert api.set_das_options(cubic=True, cf_hercules=False)
+    for sp in (2, 4, 8, 16):
+        assert rel_l2(vols[sp], vols[1]) < 1e-6
+
+
 def test_das_hercules_single_tile_no_split(torch_dev, orc):
     """A grid of one tile with few transmits takes the unsplit path."""
     print("das hercules unsplit", flush=True)
