@@ -321,7 +321,34 @@ __global__ __launch_bounds__(256) void decode_i8_kernel(const int16_t* __restric
 // kq*RPT+RPT-1 (16-byte loads; NT/8 threads cover a row segment of NT
 // samples).  Results are identical to decode_i8_kernel (same products, same
 // integer sums).
-template <int OUT_STRIDE, int NT>
+// Rows that do not start on 16 bytes (S not a multiple of 8: row k of a
+// channel starts at sample k*S): the two aligned 16-byte chunks covering a
+// thread's 8 samples are loaded through a buffer resource bounded to the
+// channel's row (R0 samples from a 16-byte aligned start, R0 a multiple of 8,
+// so a chunk is either inside the row or wholly past it and reads zeros), and
+// the 8 samples are shifted out of the 16 where they are consumed, so the
+// loads stay in flight during the previous tile's work.
+constexpr int DEC_RSRC_WORD3 = 0x00020000;  // raw buffer, 32-bit format: byte-offset range check
+
+// 8 int16 starting at sample sh (0..7) of the 16 in (lo, hi), as 4 packed pairs
+__device__ __forceinline__ v4i shift_samples(v4i lo, v4i hi, unsigned sh)
+{
+    const unsigned q = sh >> 1, rb = (sh & 1u) * 2u;  // whole dwords, then 0 or 2 bytes
+    // dword q + i of the 8 (lo then hi), i = 0..4, by a two-level select on q
+    auto pick = [&](unsigned a0, unsigned a1, unsigned a2, unsigned a3) {
+        const unsigned x = (q & 1u) ? a1 : a0;
+        const unsigned y = (q & 1u) ? a3 : a2;
+        return (q & 2u) ? y : x;
+    };
+    const unsigned d0 = lo[0], d1 = lo[1], d2 = lo[2], d3 = lo[3], d4 = hi[0], d5 = hi[1], d6 = hi[2],
+                   d7 = hi[3];
+    const unsigned e0 = pick(d0, d1, d2, d3), e1 = pick(d1, d2, d3, d4), e2 = pick(d2, d3, d4, d5),
+                   e3 = pick(d3, d4, d5, d6), e4 = pick(d4, d5, d6, d7);
+    return v4i{(int)__builtin_amdgcn_alignbyte(e1, e0, rb), (int)__builtin_amdgcn_alignbyte(e2, e1, rb),
+               (int)__builtin_amdgcn_alignbyte(e3, e2, rb), (int)__builtin_amdgcn_alignbyte(e4, e3, rb)};
+}
+
+template <int OUT_STRIDE, int NT, bool ALIGNED>
 __global__ __launch_bounds__(256, 3) void decode_i8_stream_kernel(const int16_t* __restrict__ raw,
                                                                const int16_t* __restrict__ map, size_t R0, int T,
                                                                int C, int S, int Kp, int nsx,
@@ -354,15 +381,33 @@ __global__ __launch_bounds__(256, 3) void decode_i8_stream_kernel(const int16_t*
     }
 
     // rows kq*RPT + j, samples sg*8 .. sg*8+7 of a tile, as packed int16 pairs
+    // (unaligned rows: the covering chunks cur, curh, shifted when consumed)
     v4i cur[RPT];
+    v4i curh[ALIGNED ? 1 : RPT];
     auto load_tile = [&](int tile) {
         const int c = tile / nsx, sx = tile - c * nsx;
         const int16_t* src = raw + (size_t)(uint16_t)map[c] * R0;
         const int s = sx * NT + sg * 8;
+        if constexpr (ALIGNED) {
 #pragma unroll
-        for (int j = 0; j < RPT; ++j) {
-            const int k = kq * RPT + j;
-            cur[j] = (k < T && s < S) ? *(const v4i*)(src + (size_t)k * S + s) : v4i{0, 0, 0, 0};
+            for (int j = 0; j < RPT; ++j) {
+                const int k = kq * RPT + j;
+                cur[j] = (k < T && s < S) ? *(const v4i*)(src + (size_t)k * S + s) : v4i{0, 0, 0, 0};
+            }
+        } else {
+            const __amdgpu_buffer_rsrc_t rs =
+                __builtin_amdgcn_make_buffer_rsrc(const_cast<int16_t*>(src), 0, (int)(R0 * 2), DEC_RSRC_WORD3);
+#pragma unroll
+            for (int j = 0; j < RPT; ++j) {
+                const int k = kq * RPT + j;
+                cur[j] = v4i{0, 0, 0, 0};
+                curh[j] = v4i{0, 0, 0, 0};
+                if (k < T && s < S) {
+                    const unsigned boff = (((unsigned)k * (unsigned)S + (unsigned)s) & ~7u) * 2u;
+                    cur[j] = __builtin_bit_cast(v4i, __builtin_amdgcn_raw_buffer_load_b128(rs, boff, 0, 0));
+                    curh[j] = __builtin_bit_cast(v4i, __builtin_amdgcn_raw_buffer_load_b128(rs, boff + 16u, 0, 0));
+                }
+            }
         }
     };
     int tile = blockIdx.x;
@@ -370,6 +415,14 @@ __global__ __launch_bounds__(256, 3) void decode_i8_stream_kernel(const int16_t*
     for (; tile < ntiles; tile += gridDim.x) {
         const int c = tile / nsx, s0 = (tile - c * nsx) * NT;
         __syncthreads();  // the previous tile's MFMA reads of LDS are done
+        if constexpr (!ALIGNED) {
+            const int s = s0 + sg * 8;
+#pragma unroll
+            for (int j = 0; j < RPT; ++j) {
+                const int k = kq * RPT + j;
+                cur[j] = shift_samples(cur[j], curh[j], ((unsigned)k * (unsigned)S + (unsigned)s) & 7u);
+            }
+        }
         if (kq * RPT < Kp) {
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
@@ -425,14 +478,23 @@ __global__ __launch_bounds__(256, 3) void decode_i8_stream_kernel(const int16_t*
             }
         }
         if constexpr (OUT_STRIDE == 1) {
-            // S is a multiple of 8 here, so a float4 starting below S lies inside it
             __syncthreads();
-            constexpr int Q4 = NT / 4;
-            for (int q = tid; q < T * Q4; q += 256) {
-                const int m = q / Q4, s = s0 + (q - m * Q4) * 4;
-                if (s < S)
-                    *reinterpret_cast<float4*>(out + ((size_t)m * C + c) * S + s) =
-                        *reinterpret_cast<const float4*>(&otile[m * OTS + (s - s0)]);
+            if constexpr (ALIGNED) {
+                // S is a multiple of 8 here, so a float4 starting below S lies inside it
+                constexpr int Q4 = NT / 4;
+                for (int q = tid; q < T * Q4; q += 256) {
+                    const int m = q / Q4, s = s0 + (q - m * Q4) * 4;
+                    if (s < S)
+                        *reinterpret_cast<float4*>(out + ((size_t)m * C + c) * S + s) =
+                            *reinterpret_cast<const float4*>(&otile[m * OTS + (s - s0)]);
+                }
+            } else {
+                // rows not 16-byte aligned: one float per lane, consecutive lanes
+                // on consecutive samples
+                for (int q = tid; q < T * NT; q += 256) {
+                    const int m = q / NT, sl = q - m * NT, s = s0 + sl;
+                    if (s < S) out[((size_t)m * C + c) * S + s] = otile[m * OTS + sl];
+                }
             }
         }
     }
@@ -519,7 +581,12 @@ bool launch_decode(const void* raw, int type, const int16_t* d_map, size_t R0, i
     const int Kp = (T + 31) / 32 * 32;
     dim3 grid((S + DEC_NT - 1) / DEC_NT, C, (Kp + 127) / 128);
     const bool vec = ((uintptr_t)raw % 16 == 0) && (R0 % 8 == 0) && (S % 8 == 0);
-    if (type == TYPE_I16 && mode == 1 && vec && Kp <= DEC_KC && (uintptr_t)out % 16 == 0) {
+    // streaming form: channel rows that start on 16 bytes (any S; 16-byte
+    // loads and float4 stores when S is a multiple of 8 and the output
+    // aligned, else the shifted chunks and one float per lane)
+    const bool rows_al = ((uintptr_t)raw % 16 == 0) && (R0 % 8 == 0) && R0 <= (size_t)0x3fffffff;
+    if (type == TYPE_I16 && mode == 1 && rows_al && Kp <= DEC_KC) {
+        const bool al = (S % 8 == 0) && ((uintptr_t)out % 16 == 0);
         // streaming form: a fixed grid of workgroups per CU walks the tiles
         static int cus = 0;
         if (cus == 0) {
@@ -535,10 +602,10 @@ bool launch_decode(const void* raw, int type, const int16_t* d_map, size_t R0, i
         const int ntiles = nsx * C;
         const int wgs = std::min(ntiles, cus * DEC_STREAM_WG_PER_CU);
         const int16_t* r = (const int16_t*)raw;
-        if (out_stride == 1)
-            decode_i8_stream_kernel<1, SNT><<<wgs, 256, 0, st>>>(r, d_map, R0, T, C, S, Kp, nsx, d_h8, d_rowsum, scale, out);
-        else
-            decode_i8_stream_kernel<2, SNT><<<wgs, 256, 0, st>>>(r, d_map, R0, T, C, S, Kp, nsx, d_h8, d_rowsum, scale, out);
+#define CUT_DEC_S(O, A) decode_i8_stream_kernel<O, SNT, A><<<wgs, 256, 0, st>>>(r, d_map, R0, T, C, S, Kp, nsx, d_h8, d_rowsum, scale, out)
+        if (al) { if (out_stride == 1) CUT_DEC_S(1, true); else CUT_DEC_S(2, true); }
+        else { if (out_stride == 1) CUT_DEC_S(1, false); else CUT_DEC_S(2, false); }
+#undef CUT_DEC_S
     } else if (type == TYPE_I16 && mode == 1) {
         const int16_t* r = (const int16_t*)raw;
 #define CUT_DEC_I8(V, O) decode_i8_kernel<V, O><<<grid, 256, 0, st>>>(r, d_map, R0, T, C, S, Kp, d_h8, d_rowsum, scale, out)

@@ -30,7 +30,10 @@ def _rf_setup(acq):
 
 
 @pytest.mark.parametrize("T,S,encoding", [(32, 1024, L.HADAMARD), (128, 512, L.HADAMARD), (16, 256, L.WALSH),
-                                          (64, 200, L.HADAMARD), (24, 128, L.HADAMARD), (40, 96, L.WALSH)])
+                                          (64, 200, L.HADAMARD), (24, 128, L.HADAMARD), (40, 96, L.WALSH),
+                                          # rows that do not start on 16 bytes (S not a multiple of 8)
+                                          (32, 1001, L.HADAMARD), (128, 1027, L.HADAMARD), (16, 3001, L.WALSH),
+                                          (64, 1500, L.HADAMARD), (128, 2050, L.HADAMARD)])
 def test_decode_bit_exact(torch_dev, orc, T, S, encoding):
     import torch
     print(f"decode T={T} S={S} enc={encoding}", flush=True)
