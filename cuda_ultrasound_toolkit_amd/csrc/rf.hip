@@ -488,9 +488,25 @@ __global__ __launch_bounds__(256, 3) void decode_i8_stream_kernel(const int16_t*
                         *reinterpret_cast<float4*>(out + ((size_t)m * C + c) * S + s) =
                             *reinterpret_cast<const float4*>(&otile[m * OTS + (s - s0)]);
                 }
+            } else if (S % 4 == 0 && (uintptr_t)out % 16 == 0) {
+                // rows of a multiple of 4 samples: float4 segments as above
+                constexpr int Q4 = NT / 4;
+                for (int q = tid; q < T * Q4; q += 256) {
+                    const int m = q / Q4, s = s0 + (q - m * Q4) * 4;
+                    if (s < S)
+                        *reinterpret_cast<float4*>(out + ((size_t)m * C + c) * S + s) =
+                            *reinterpret_cast<const float4*>(&otile[m * OTS + (s - s0)]);
+                }
+            } else if (S % 2 == 0 && (uintptr_t)out % 8 == 0) {
+                constexpr int Q2 = NT / 2;
+                for (int q = tid; q < T * Q2; q += 256) {
+                    const int m = q / Q2, s = s0 + (q - m * Q2) * 2;
+                    if (s < S)
+                        *reinterpret_cast<float2*>(out + ((size_t)m * C + c) * S + s) =
+                            *reinterpret_cast<const float2*>(&otile[m * OTS + (s - s0)]);
+                }
             } else {
-                // rows not 16-byte aligned: one float per lane, consecutive lanes
-                // on consecutive samples
+                // one float per lane, consecutive lanes on consecutive samples
                 for (int q = tid; q < T * NT; q += 256) {
                     const int m = q / NT, sl = q - m * NT, s = s0 + sl;
                     if (s < S) out[((size_t)m * C + c) * S + s] = otile[m * OTS + sl];
