@@ -2,7 +2,7 @@
 
 The 128 tx x 128 ch x 4096 frame (64-tap match filter, linear interpolation,
 F# 1.5 Hann) is beamformed to the whole 128^3 volume through
-cut_beamform_device exactly as the benchmark does (tuned transmit split and
+cut_beamform_device exactly as the benchmark does (16-way transmit split and
 all); depth planes 16, 48, 80 and 112 of that volume are compared with the
 oracle evaluated on the same planes of the same grid.  Bar: relative L2 below
 1e-4 (FP32 against FP64) on every seed.
@@ -33,7 +33,7 @@ def test_full_volume_planes_match_oracle(torch_dev, orc, seed):
         d_raw = torch.from_numpy(acq.raw).to(torch_dev)
         vol = torch.zeros(n[0] * n[1] * n[2] * 2, dtype=torch.float32, device=torch_dev)
         assert api.beamform_device(d_raw, vol, bp)
-        assert api.beamform_device(d_raw, vol, bp)  # the tuned launch (the first call tunes the split)
+        assert api.beamform_device(d_raw, vol, bp)  # a second call, as the benchmark's timed steps
         torch.cuda.synchronize()
     finally:
         assert api.set_das_options(cubic=True)
