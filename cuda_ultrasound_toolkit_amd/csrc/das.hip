@@ -438,7 +438,7 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
               // (the largest |dx| and dy^2, same FP32 expression, monotonic), and
               // the patch box bounds the smallest from below.
               unsigned long long m_any64 = 0, m_full64 = 0;
-              if constexpr (!SINAPO && !CF) {
+              if constexpr (!SINAPO) {
                   const float dxa = my_ex - wvx0, dxb = my_ex - wvx1;
                   const float dx_hi = fmaxf(fabsf(dxa), fabsf(dxb));
                   const bool ex_in = my_ex >= fminf(wvx0, wvx1) && my_ex <= fmaxf(wvx0, wvx1);
@@ -572,9 +572,9 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
                     if constexpr (CF) inc[j] = fmaf(v.x, v.x, fmaf(v.y, v.y, inc[j]));  // w != 0 here
                 };
                 auto chan_ex = [&](int cc) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(my_ex), sub + cc)); };
-                if constexpr (!SINAPO && !CF) {
-                    // Hann apodization without the coherence factor.  The pass's
-                    // channels are sorted first into two scalar masks: those where
+                if constexpr (!SINAPO) {
+                    // Hann apodization (with or without the coherence factor).  The
+                    // pass's channels are sorted first into two scalar masks: those where
                     // every lane of the wave is inside the aperture at every depth of
                     // the tile (rho2 < r2_full; most wave-channels of a volume), whose
                     // taps need no threshold test, and those where only some are
@@ -622,6 +622,13 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
                                 v = make_float2(fmaf(tt, cur.e0.z, cur.e0.x), fmaf(tt, cur.e0.w, cur.e0.y));
                             acc[j].x = fmaf(w, v.x, acc[j].x);
                             acc[j].y = fmaf(w, v.y, acc[j].y);
+                            // the incoherent sum takes the taps inside the aperture only
+                            // (a weight that passes the threshold is at least 0.1)
+                            if constexpr (CF) {
+                                const float in = fmaf(v.x, v.x, fmaf(v.y, v.y, inc[j]));
+                                if constexpr (decltype(thr)::value) inc[j] = w != 0.0f ? in : inc[j];
+                                else inc[j] = in;
+                            }
                             if (j + 1 < HT_Z) cur = nxt;
                         }
                     };
@@ -640,8 +647,9 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
                         taps8(wb, fmaf(dx, dx, dy2), std::true_type{});
                     }
                 } else {
-                    // channels whose window sits in LDS; a single path keeps the
-                    // accumulators in place across iterations
+                    // generic apodization (sine, or the exact Hann weight of volumes
+                    // reaching z <= 0): channels whose window sits in LDS; a single
+                    // path keeps the accumulators in place across iterations
                     for (int cc = 0; cc < nC; ++cc) {
                         if (!((lds_mask >> cc) & 1ull)) continue;
                         const int wb = __builtin_amdgcn_readlane(my_base, cc);
