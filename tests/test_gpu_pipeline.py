@@ -340,6 +340,20 @@ def test_das_hercules_coherence_factor_extension(torch_dev, orc):
 
 
 @pytest.mark.parametrize("cubic", [False, True])
+@pytest.mark.parametrize("focus", [False, True])
+def test_das_hercules_coherence_factor_loops(torch_dev, orc, cubic, focus):
+    """The coherence factor in the classified, pipelined tap loops (wave-channels
+    wholly inside the aperture and partly inside, the incoherent sum over the
+    taps inside only): linear and cubic, plane and focused transmits, on a
+    ragged grid."""
+    print(f"das hercules cf loops cubic={cubic} focus={focus}", flush=True)
+    acq = configs.rca_small(seed=31, n=21, **({"focus_z": 8e-3} if focus else {}))
+    acq.bp.output_points[:] = (21, 17, 23, 1)
+    acq.bp.coherency_weighting = 0.6
+    _das_case(acq, orc, torch_dev, cubic, 1e-4, cf=True)
+
+
+@pytest.mark.parametrize("cubic", [False, True])
 def test_das_hercules_every_transmit_split(torch_dev, orc, tmp_path, cubic):
     """Every transmit split of the HERCULES launch (1, 2, 4, 8 and 16 workgroups
     per tile, forced through the tuning table for this shape) against the
