@@ -2,7 +2,7 @@
 
 Bars: decode bit-exact; convert exact; Hilbert within 1e-5 relative L2;
 DAS and the whole beamform within 1e-4 relative L2 (FP32 arithmetic against
-FP64, the north-star tolerance for beamformed IQ).
+FP64, the tolerance set for beamformed IQ).
 """
 import numpy as np
 import pytest
