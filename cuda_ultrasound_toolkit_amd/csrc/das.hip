@@ -286,10 +286,10 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
         }
         base_d[j] = (txd + (double)fz) * (double)p.spm + (double)p.delay;
         base[j] = (float)base_d[j];
-        // txd = z: depth-only, held in scalar registers (24 VGPRs fewer).  Linear
-        // only: the cubic kernel measured slower so (12.3 against 11.8 ms on the
-        // 128^3 frame), although it then no longer spills.
-        if constexpr (!FOCUS && !CUBIC) {
+        // txd = z: depth-only, held in scalar registers (24 VGPRs fewer; the
+        // cubic kernel with its pipelined taps 10.27 -> 10.20 ms on the 128^3
+        // frame, where before the pipelining it measured slower so)
+        if constexpr (!FOCUS) {
             base[j] = uni(base[j]);
             base_d[j] = uni_d(base_d[j]);
         }
