@@ -138,6 +138,11 @@ __device__ __forceinline__ float2 interp_global(const void* __restrict__ base, s
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
 
+// One weight for every tap of a channel (the sine apodization).
+struct ConstW {
+    float w;
+};
+
 // Weight of one tap: apodization when it passes the reference's threshold
 // (apo > 0.1), else 0.  (Lanes of voxels outside the grid compute in-range
 // values that are never stored.)
@@ -438,6 +443,7 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
               // (the largest |dx| and dy^2, same FP32 expression, monotonic), and
               // the patch box bounds the smallest from below.
               unsigned long long m_any64 = 0, m_full64 = 0;
+              float my_apo = 0.0f;  // sine apodization: lane L's channel weight
               if constexpr (!SINAPO) {
                   const float dxa = my_ex - wvx0, dxb = my_ex - wvx1;
                   const float dx_hi = fmaxf(fabsf(dxa), fabsf(dxb));
@@ -447,6 +453,11 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
                   const bool all_in = fmaf(dx_hi, dx_hi, wdy2_hi) < r2_full;
                   m_any64 = __builtin_amdgcn_ballot_w64(any_in);
                   m_full64 = __builtin_amdgcn_ballot_w64(all_in) & m_any64;
+              } else if (!hann_exact) {
+                  // sine apodization of channel cb64 + lane (as chan() forms it)
+                  my_apo = sin_apo_to(my_ex, p.xdc_max_x, p.to_power) * apo_y;
+                  m_full64 = __builtin_amdgcn_ballot_w64(lane < nC64 && my_apo > 0.1f);
+                  m_any64 = m_full64;
               }
               __syncthreads();  // tables ready
               for (int sub = 0; sub < nC64; sub += NCB) {
@@ -572,6 +583,57 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
                     if constexpr (CF) inc[j] = fmaf(v.x, v.x, fmaf(v.y, v.y, inc[j]));  // w != 0 here
                 };
                 auto chan_ex = [&](int cc) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(my_ex), sub + cc)); };
+                // The 8 taps of a channel are software-pipelined: the window
+                // entry (entries, cubic) of depth j+1 is requested before depth j
+                // is accumulated, so each wave keeps LDS reads in flight across
+                // its arithmetic (each tap waiting on its own read left the
+                // kernel latency-bound at four waves per SIMD).  mode: std::false_type
+                // (Hann weight, every tap inside the aperture), std::true_type (Hann
+                // weight and threshold), ConstW (one weight for the channel: sine
+                // apodization).
+                auto taps8 = [&](int wb, float rho2, auto mode) {
+                    struct Ent {
+                        float4 e0, e1;
+                        float tt;
+                    };
+                    auto fetch = [&](int j) {
+                        Ent r;
+                        const float xr = fmaf(fsqrt(rho2 + z2[j]), p.spm, base_r[j]);
+                        const float x = CLAMP ? __builtin_amdgcn_fmed3f(xr, lo_r, hi_r) : xr;
+                        r.tt = __builtin_amdgcn_fractf(x);
+                        const int addr = wb + (int)x * (int)WIN_ESZ;
+                        r.e0 = *reinterpret_cast<const float4*>(wbytes + addr);
+                        if constexpr (CUBIC) r.e1 = *reinterpret_cast<const float4*>(wbytes + addr + HPOOL * WIN_ESZ);
+                        return r;
+                    };
+                    Ent cur = fetch(0);
+#pragma unroll
+                    for (int j = 0; j < HT_Z; ++j) {
+                        Ent nxt;
+                        if (j + 1 < HT_Z) nxt = fetch(j + 1);
+                        float w;
+                        if constexpr (std::is_same_v<decltype(mode), ConstW>) w = mode.w;
+                        else if constexpr (decltype(mode)::value) w = tap_weight<false>(rho2, kz2[j], 0.0f);
+                        else w = hann_poly(rho2 * kz2[j]);
+                        const float tt = cur.tt;
+                        float2 v;
+                        if constexpr (CUBIC)
+                            v = make_float2(fmaf(fmaf(fmaf(cur.e1.z, tt, cur.e1.x), tt, cur.e0.z), tt, cur.e0.x),
+                                            fmaf(fmaf(fmaf(cur.e1.w, tt, cur.e1.y), tt, cur.e0.w), tt, cur.e0.y));
+                        else
+                            v = make_float2(fmaf(tt, cur.e0.z, cur.e0.x), fmaf(tt, cur.e0.w, cur.e0.y));
+                        acc[j].x = fmaf(w, v.x, acc[j].x);
+                        acc[j].y = fmaf(w, v.y, acc[j].y);
+                        // the incoherent sum takes the taps inside the aperture only
+                        // (a weight that passes the threshold is at least 0.1)
+                        if constexpr (CF) {
+                            const float in = fmaf(v.x, v.x, fmaf(v.y, v.y, inc[j]));
+                            if constexpr (std::is_same_v<decltype(mode), std::true_type>) inc[j] = w != 0.0f ? in : inc[j];
+                            else inc[j] = in;
+                        }
+                        if (j + 1 < HT_Z) cur = nxt;
+                    }
+                };
                 if constexpr (!SINAPO) {
                     // Hann apodization (with or without the coherence factor).  The
                     // pass's channels are sorted first into two scalar masks: those where
@@ -585,53 +647,6 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
                     const unsigned long long m_any = (m_any64 >> sub) & lds_mask;
                     unsigned long long m_full = (m_full64 >> sub) & m_any;
                     unsigned long long m_part = m_any & ~m_full;
-                    // The 8 taps of a channel are software-pipelined: the window
-                    // entry (entries, cubic) of depth j+1 is requested before depth j
-                    // is accumulated, so each wave keeps LDS reads in flight across
-                    // its arithmetic (each tap waiting on its own read left the
-                    // kernel latency-bound at four waves per SIMD).
-                    auto taps8 = [&](int wb, float rho2, auto thr) {
-                        struct Ent {
-                            float4 e0, e1;
-                            float tt;
-                        };
-                        auto fetch = [&](int j) {
-                            Ent r;
-                            const float xr = fmaf(fsqrt(rho2 + z2[j]), p.spm, base_r[j]);
-                            const float x = CLAMP ? __builtin_amdgcn_fmed3f(xr, lo_r, hi_r) : xr;
-                            r.tt = __builtin_amdgcn_fractf(x);
-                            const int addr = wb + (int)x * (int)WIN_ESZ;
-                            r.e0 = *reinterpret_cast<const float4*>(wbytes + addr);
-                            if constexpr (CUBIC) r.e1 = *reinterpret_cast<const float4*>(wbytes + addr + HPOOL * WIN_ESZ);
-                            return r;
-                        };
-                        Ent cur = fetch(0);
-#pragma unroll
-                        for (int j = 0; j < HT_Z; ++j) {
-                            Ent nxt;
-                            if (j + 1 < HT_Z) nxt = fetch(j + 1);
-                            float w;
-                            if constexpr (decltype(thr)::value) w = tap_weight<false>(rho2, kz2[j], 0.0f);
-                            else w = hann_poly(rho2 * kz2[j]);
-                            const float tt = cur.tt;
-                            float2 v;
-                            if constexpr (CUBIC)
-                                v = make_float2(fmaf(fmaf(fmaf(cur.e1.z, tt, cur.e1.x), tt, cur.e0.z), tt, cur.e0.x),
-                                                fmaf(fmaf(fmaf(cur.e1.w, tt, cur.e1.y), tt, cur.e0.w), tt, cur.e0.y));
-                            else
-                                v = make_float2(fmaf(tt, cur.e0.z, cur.e0.x), fmaf(tt, cur.e0.w, cur.e0.y));
-                            acc[j].x = fmaf(w, v.x, acc[j].x);
-                            acc[j].y = fmaf(w, v.y, acc[j].y);
-                            // the incoherent sum takes the taps inside the aperture only
-                            // (a weight that passes the threshold is at least 0.1)
-                            if constexpr (CF) {
-                                const float in = fmaf(v.x, v.x, fmaf(v.y, v.y, inc[j]));
-                                if constexpr (decltype(thr)::value) inc[j] = w != 0.0f ? in : inc[j];
-                                else inc[j] = in;
-                            }
-                            if (j + 1 < HT_Z) cur = nxt;
-                        }
-                    };
                     while (m_full) {
                         const int cc = __builtin_ctzll(m_full);
                         m_full &= m_full - 1;
@@ -646,32 +661,34 @@ __device__ __forceinline__ void hercules_tile(const void* __restrict__ iq, float
                         const float dx = chan_ex(cc) - vx;
                         taps8(wb, fmaf(dx, dx, dy2), std::true_type{});
                     }
+                } else if (!hann_exact) {
+                    // sine apodization: one weight per channel (and transmit), the same
+                    // for every lane and depth; channels whose weight passes 0.1
+                    unsigned long long m_full = (m_full64 >> sub) & lds_mask;
+                    while (m_full) {
+                        const int cc = __builtin_ctzll(m_full);
+                        m_full &= m_full - 1;
+                        const int wb = __builtin_amdgcn_readlane(my_base, cc);
+                        const float dx = chan_ex(cc) - vx;
+                        const float wc = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(my_apo), sub + cc));
+                        taps8(wb, fmaf(dx, dx, dy2), ConstW{wc});
+                    }
                 } else {
-                    // generic apodization (sine, or the exact Hann weight of volumes
-                    // reaching z <= 0): channels whose window sits in LDS; a single
-                    // path keeps the accumulators in place across iterations
+                    // the exact Hann weight of volumes reaching z <= 0: channels whose
+                    // window sits in LDS; a single path keeps the accumulators in place
+                    // across iterations
                     for (int cc = 0; cc < nC; ++cc) {
                         if (!((lds_mask >> cc) & 1ull)) continue;
                         const int wb = __builtin_amdgcn_readlane(my_base, cc);
-                        float rho2, apo_c;
-                        if (!chan(chan_ex(cc), rho2, apo_c)) continue;
+                        const float dx = chan_ex(cc) - vx;
+                        const float rho2 = fmaf(dx, dx, dy2);
                         // Taps outside the aperture get weight 0, or, with the coherence
                         // factor, are masked off (and skipped when no lane of the wave is
-                        // inside), which measured faster there (and slower without it).
-                        // chan() already required apo_c > 0.1.
+                        // inside).
 #pragma unroll
                         for (int j = 0; j < HT_Z; ++j) {
-                            float w;
-                            if constexpr (SINAPO) {
-                                w = hann_exact ? hann_weight_any(rho2, kzs[j]) : apo_c;
-                                if (CF && w == 0.0f) continue;
-                            } else if constexpr (CF) {
-                                const float s = rho2 * kz2[j];
-                                if (!(s < HANN_S_THR)) continue;
-                                w = hann_poly(s);
-                            } else {
-                                w = tap_weight<false>(rho2, kz2[j], apo_c);
-                            }
+                            const float w = hann_weight_any(rho2, kzs[j]);
+                            if (CF && w == 0.0f) continue;
                             tap_at(j, w, wb, rho2);
                         }
                     }

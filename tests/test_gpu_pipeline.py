@@ -443,12 +443,26 @@ def test_das_hercules_above_array_cf_and_negative_fnumber(torch_dev, orc):
     _das_case(acq, orc, torch_dev, True, 1e-4)
 
 
-def test_das_hercules_sin_apodization(torch_dev, orc):
-    print("das hercules RX_TO_SIN", flush=True)
-    acq = configs.rca_small(seed=23)
+@pytest.mark.parametrize("cubic,cf,focus", [(True, False, None), (False, False, None), (True, True, None),
+                                            (False, True, None), (True, False, 8e-3), (False, True, 8e-3)])
+def test_das_hercules_sin_apodization(torch_dev, orc, cubic, cf, focus):
+    print(f"das hercules RX_TO_SIN cubic={cubic} cf={cf} focus={focus}", flush=True)
+    acq = configs.rca_small(seed=23) if focus is None else configs.rca_small(seed=25, focus_z=focus)
     acq.bp.apo_type = L.RX_TO_SIN
     acq.bp.to_power = 2
-    _das_case(acq, orc, torch_dev, True, 1e-4)
+    if cf:
+        acq.bp.coherency_weighting = 0.5
+    _das_case(acq, orc, torch_dev, cubic, 1e-4, cf=cf)
+
+
+def test_das_hercules_sin_apodization_above_array(torch_dev, orc):
+    """Sine apodization over a grid reaching z <= 0: the weight depends on the
+    element alone, so the volume takes the same channel loop as below the array."""
+    print("das hercules RX_TO_SIN above array", flush=True)
+    acq = configs.rca_small(seed=24, z_range=(-3.125e-3, 8.875e-3))
+    acq.bp.apo_type = L.RX_TO_SIN
+    acq.bp.to_power = 2
+    _das_case(acq, orc, torch_dev, False, 1e-4)
 
 
 @pytest.mark.parametrize("G,gid,enc", [(2, 1, L.HADAMARD), (4, 2, L.HADAMARD), (4, 3, L.WALSH)])
