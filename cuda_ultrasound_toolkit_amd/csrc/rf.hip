@@ -62,14 +62,30 @@ bool launch_convert(const void* raw, int type, const int16_t* d_map, size_t R0, 
 // both sides) into LDS, then each output is the FIR over the taps that fall
 // inside the row, divided by their count (taps are finite).  The phase s fd/fs is reduced to
 // revolutions in FP64 and evaluated with v_sin/v_cos.  LDS holds the samples
-// from org = s0 - L/2 on, zeros where the window leaves the row, so every
-// output sums all L taps (the zeros add nothing) in tap order, as the valid
-// taps alone would be summed.  The samples sit in four phase planes, sample
-// org + j at plane j % 4, entry j / 4.  Each thread computes four consecutive
-// outputs, sliding a four-sample register window along the row: four taps
-// per step, one read per plane, consecutive entries across the lanes.
+// from org = s0 - L/2 on, zeros where the window leaves the row (and past
+// its end, as far as the products below read), so every output sums all L
+// taps (the zeros add nothing).
+//
+// The FIR runs on the matrix cores as a banded (Toeplitz) product.  Outputs
+// go in blocks of 16 consecutive samples; output 16 b + m of the segment is
+// sum_k T[m][k] X[k][b] with X[k][b] = v[16 b + k] (the window from the
+// block's first sample on) and T[m][k] = hr[k - m], zero unless
+// 0 <= k - m < L (hr = the taps in application order, hr[i] = h[L-1-i]).
+// A wave computes 16 blocks (256 outputs) with v_mfma_f32_16x16x4_f32 over
+// K = L + 15 rounded up to 16 (80 for 64 taps: 80 % of the products are
+// taps), the I and Q parts as two products sharing the tap operand, the
+// operands of four K steps read from LDS ahead of their eight products.  The
+// FP32 matrix rate is twice the non-packed FMA rate; every product is the
+// same single-rounded FP32 product as an FMA's, summed in another order.
+// Sample j sits at LDS entry j + j/16, so the 16 blocks a B operand spans
+// (16 samples apart) fall on different banks.  Workgroups go transmit-fastest
+// (t = blockIdx.x % T): consecutive workgroups read consecutive stretches of
+// a channel's raw row.
 constexpr int BP_SEG = 1024;
-constexpr int BP_VTS = (BP_SEG + MAX_FILTER_LENGTH) / 4 + 1;  // entries per phase plane
+constexpr int BP_WIN = BP_SEG + MAX_FILTER_LENGTH + 16;  // window samples read (at most BP_SEG + L + 14)
+constexpr int BP_VT = BP_WIN + BP_WIN / 16 + 1;         // LDS entries of the window
+constexpr int BP_HP = MAX_FILTER_LENGTH + 48;           // tap i at 16 + i, zeros around (read: 1 .. L + 45)
+typedef float v4f __attribute__((ext_vector_type(4)));
 
 template <typename T>
 __global__ __launch_bounds__(256) void convert_demod_kernel(const T* __restrict__ raw, const int16_t* __restrict__ map,
@@ -77,21 +93,24 @@ __global__ __launch_bounds__(256) void convert_demod_kernel(const T* __restrict_
                                                             const float* __restrict__ h, int L,
                                                             float2* __restrict__ out)
 {
-    static_assert(BP_SEG == 4 * 256, "four outputs per thread");
-    __shared__ float2 vt[4 * BP_VTS];
-    __shared__ __align__(16) float hr[MAX_FILTER_LENGTH];  // taps in application order: hr[i] = h[L-1-i]
-    const int row = blockIdx.x;              // t * C + c
-    const int t = row / C, c = row - t * C;
+    static_assert(BP_SEG == 4 * 256, "four waves of 16 blocks of 16 outputs");
+    __shared__ float2 vt[BP_VT];
+    __shared__ float hp[BP_HP];
+    const int nT = gridDim.x / C;
+    const int t = blockIdx.x % nT, c = blockIdx.x / nT;
+    const int row = t * C + c;               // output row
     const int s0 = blockIdx.y * BP_SEG;
     const int half = L > 0 ? L / 2 : 0;
     const int org = s0 - half;
     const int nwin = BP_SEG + (L > 0 ? L - 1 : 0);  // window samples org .. org + nwin - 1
+    const int nfill = BP_SEG + (L > 0 ? L + 14 : 0);  // entries written (zeros past nwin)
+    const int Kp = (L + 30) & ~15;                    // K = L + 15 rounded up to 16
     const int tid = threadIdx.x;
     // sample org + j of the row (j >= 0)
-    auto v = [&](int j) -> float2& { return vt[(j & 3) * BP_VTS + (j >> 2)]; };
+    auto v = [&](int j) -> float2& { return vt[j + (j >> 4)]; };
     const T* r = raw + (size_t)(uint16_t)map[c] * R0 + 2 * (size_t)t * S;
     // every thread issues all its loads first
-    constexpr int FILL = (BP_SEG + MAX_FILTER_LENGTH) / 256;
+    constexpr int FILL = (BP_SEG + MAX_FILTER_LENGTH + 14 + 255) / 256;
     struct alignas(2 * sizeof(T)) IQPair { T i, q; };
     // (I, Q) as one load when the row start allows it (uniform per workgroup)
     const bool paired = (reinterpret_cast<uintptr_t>(r) & (2 * sizeof(T) - 1)) == 0;
@@ -114,9 +133,9 @@ __global__ __launch_bounds__(256) void convert_demod_kernel(const T* __restrict_
 #pragma unroll
     for (int u = 0; u < FILL; ++u) {
         const int j = tid + 256 * u, js = org + j;
-        if (j < nwin) {
+        if (j < nfill) {
             float2 y = make_float2(0.0f, 0.0f);
-            if (js >= 0 && js < S) {
+            if (j < nwin && js >= 0 && js < S) {
                 const double ph = (double)js * fd_fs;
                 const float f = (float)(ph - floor(ph));
                 const float cs = __builtin_amdgcn_cosf(f), sn = __builtin_amdgcn_sinf(f);
@@ -126,54 +145,48 @@ __global__ __launch_bounds__(256) void convert_demod_kernel(const T* __restrict_
             v(j) = y;
         }
     }
-    for (int i = tid; i < L; i += blockDim.x) hr[i] = h[L - 1 - i];
+    if (L > 0)
+        for (int i = tid; i < L + 46; i += blockDim.x) hp[i] = (i >= 16 && i < 16 + L) ? h[L - 1 - (i - 16)] : 0.0f;
     __syncthreads();
-    const int sb = s0 + 4 * tid;
-    if (sb >= S) return;
     float2* const orow = out + (size_t)row * S;
-    const int nout = min(4, S - sb);
     if (L <= 0) {
-        for (int q = 0; q < nout; ++q) orow[sb + q] = v(4 * tid + q);
+        const int sb = s0 + 4 * tid;
+        for (int q = 0; q < 4 && sb + q < S; ++q) orow[sb + q] = v(4 * tid + q);
         return;
     }
-    // output sb + q takes sample org + 4 tid + q + i at tap i
-    const float2* p0 = vt + 0 * BP_VTS + tid;
-    const float2* p1 = vt + 1 * BP_VTS + tid;
-    const float2* p2 = vt + 2 * BP_VTS + tid;
-    const float2* p3 = vt + 3 * BP_VTS + tid;
-    float2 w0 = p0[0], w1 = p1[0], w2 = p2[0];
-    float ax0 = 0.0f, ay0 = 0.0f, ax1 = 0.0f, ay1 = 0.0f, ax2 = 0.0f, ay2 = 0.0f, ax3 = 0.0f, ay3 = 0.0f;
-    auto tap = [&](float hv, float2 a, float2 b, float2 c_, float2 d) {
-        ax0 = fmaf(a.x, hv, ax0); ay0 = fmaf(a.y, hv, ay0);
-        ax1 = fmaf(b.x, hv, ax1); ay1 = fmaf(b.y, hv, ay1);
-        ax2 = fmaf(c_.x, hv, ax2); ay2 = fmaf(c_.y, hv, ay2);
-        ax3 = fmaf(d.x, hv, ax3); ay3 = fmaf(d.y, hv, ay3);
-    };
-    const int L4 = L & ~3;
-    int i = 0;
-    for (int k = 0; i < L4; ++k, i += 4) {
-        const float4 h4 = *reinterpret_cast<const float4*>(hr + i);
-        const float2 w3 = p3[k], w4 = p0[k + 1], w5 = p1[k + 1], w6 = p2[k + 1];
-        tap(h4.x, w0, w1, w2, w3);
-        tap(h4.y, w1, w2, w3, w4);
-        tap(h4.z, w2, w3, w4, w5);
-        tap(h4.w, w3, w4, w5, w6);
-        w0 = w4; w1 = w5; w2 = w6;
+    const int lane = tid & 63;
+    if (s0 + 256 * (tid >> 6) >= S) return;         // the wave's outputs are past the row (uniform)
+    const int b = (tid >> 6) * 16 + (lane & 15);    // the lane's block: B and D column
+    const int kr = lane >> 4;                       // the lane's K row of A and B
+    const float* hl = hp + 16 + kr - (lane & 15);   // A[m][kr] = T[m][k0 + kr], m = lane & 15
+    int j = 16 * b + kr;                            // B[kr][b] = X[k0 + kr][b]
+    v4f yi = {0.0f, 0.0f, 0.0f, 0.0f}, yq = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int k0 = 0; k0 < Kp; k0 += 16, j += 16) {
+        float a[4];
+        float2 x[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            a[u] = hl[k0 + 4 * u];
+            x[u] = v(j + 4 * u);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            yi = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u], x[u].x, yi, 0, 0, 0);
+            yq = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u], x[u].y, yq, 0, 0, 0);
+        }
     }
-    for (; i < L; ++i) {
-        const float2 w3 = v(4 * tid + i + 3);
-        tap(hr[i], w0, w1, w2, w3);
-        w0 = w1; w1 = w2; w2 = w3;
-    }
+    // D[m][b], m = 4 kr + q: the lane holds outputs 16 b + 4 kr + q, q = 0..3
+    const int sb = s0 + 16 * b + 4 * kr;
+    if (sb >= S) return;
+    const int nout = min(4, S - sb);
     // divided by the number of taps inside the row: i with 0 <= s - half + i < S
-    const float ax[4] = {ax0, ax1, ax2, ax3}, ay[4] = {ay0, ay1, ay2, ay3};
     float2 y[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const int s = sb + q;
         const int ia = max(0, half - s), ib = min(L - 1, S - 1 - s + half);
         const float inv = 1.0f / (float)max(1, ib - ia + 1);
-        y[q] = make_float2(ax[q] * inv, ay[q] * inv);
+        y[q] = make_float2(yi[q] * inv, yq[q] * inv);
     }
     if (nout == 4 && (reinterpret_cast<uintptr_t>(orow + sb) & 15) == 0) {
         float4* o4 = reinterpret_cast<float4*>(orow + sb);
