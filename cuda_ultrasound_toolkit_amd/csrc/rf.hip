@@ -82,9 +82,11 @@ bool launch_convert(const void* raw, int type, const int16_t* d_map, size_t R0, 
 // (t = blockIdx.x % T): consecutive workgroups read consecutive stretches of
 // a channel's raw row.
 constexpr int BP_SEG = 1024;
-constexpr int BP_WIN = BP_SEG + MAX_FILTER_LENGTH + 16;  // window samples read (at most BP_SEG + L + 14)
-constexpr int BP_VT = BP_WIN + BP_WIN / 16 + 1;         // LDS entries of the window
-constexpr int BP_HP = MAX_FILTER_LENGTH + 48;           // tap i at 16 + i, zeros around (read: 1 .. L + 45)
+// LDS (sized for the filter at launch): the window's entries, samples
+// 0 .. BP_SEG + L + 13 (the last read is BP_SEG + L + 13 at most), then the
+// taps at 16 .. 16 + L - 1 with zeros around (reads 1 .. L + 45)
+__host__ __device__ inline int bp_window(int L) { return BP_SEG + (L > 0 ? L + 14 : 0); }
+__host__ __device__ inline int bp_entries(int n) { return n + n / 16 + 1; }
 typedef float v4f __attribute__((ext_vector_type(4)));
 
 template <typename T>
@@ -94,8 +96,9 @@ __global__ __launch_bounds__(256) void convert_demod_kernel(const T* __restrict_
                                                             float2* __restrict__ out)
 {
     static_assert(BP_SEG == 4 * 256, "four waves of 16 blocks of 16 outputs");
-    __shared__ float2 vt[BP_VT];
-    __shared__ float hp[BP_HP];
+    extern __shared__ float2 bp_lds[];
+    float2* const vt = bp_lds;
+    float* const hp = reinterpret_cast<float*>(bp_lds + bp_entries(bp_window(L)));
     const int nT = gridDim.x / C;
     const int t = blockIdx.x % nT, c = blockIdx.x / nT;
     const int row = t * C + c;               // output row
@@ -103,7 +106,7 @@ __global__ __launch_bounds__(256) void convert_demod_kernel(const T* __restrict_
     const int half = L > 0 ? L / 2 : 0;
     const int org = s0 - half;
     const int nwin = BP_SEG + (L > 0 ? L - 1 : 0);  // window samples org .. org + nwin - 1
-    const int nfill = BP_SEG + (L > 0 ? L + 14 : 0);  // entries written (zeros past nwin)
+    const int nfill = bp_window(L);                  // samples written (zeros past nwin)
     const int Kp = (L + 30) & ~15;                    // K = L + 15 rounded up to 16
     const int tid = threadIdx.x;
     // sample org + j of the row (j >= 0)
@@ -114,6 +117,14 @@ __global__ __launch_bounds__(256) void convert_demod_kernel(const T* __restrict_
     struct alignas(2 * sizeof(T)) IQPair { T i, q; };
     // (I, Q) as one load when the row start allows it (uniform per workgroup)
     const bool paired = (reinterpret_cast<uintptr_t>(r) & (2 * sizeof(T) - 1)) == 0;
+    // the taps (entries 16 .. 16 + L - 1 of hp) load with the samples
+    constexpr int HFILL = (MAX_FILTER_LENGTH + 46 + 255) / 256;
+    float hv[HFILL];
+#pragma unroll
+    for (int u = 0; u < HFILL; ++u) {
+        const int i = tid + 256 * u;
+        hv[u] = (L > 0 && i >= 16 && i < 16 + L) ? h[L - 1 - (i - 16)] : 0.0f;
+    }
     float Iv[FILL], Qv[FILL];
 #pragma unroll
     for (int u = 0; u < FILL; ++u) {
@@ -146,7 +157,9 @@ __global__ __launch_bounds__(256) void convert_demod_kernel(const T* __restrict_
         }
     }
     if (L > 0)
-        for (int i = tid; i < L + 46; i += blockDim.x) hp[i] = (i >= 16 && i < 16 + L) ? h[L - 1 - (i - 16)] : 0.0f;
+#pragma unroll
+        for (int u = 0; u < HFILL; ++u)
+            if (tid + 256 * u < L + 46) hp[tid + 256 * u] = hv[u];
     __syncthreads();
     float2* const orow = out + (size_t)row * S;
     if (L <= 0) {
@@ -205,10 +218,11 @@ bool launch_convert_demod(const void* raw, int type, const int16_t* d_map, size_
     if (L < 0 || L > MAX_FILTER_LENGTH || L > S) CUT_FAIL("bandpass filter length " << L << " not in [0, min(S, " << MAX_FILTER_LENGTH << ")]");
     if (L > 0 && !d_h) CUT_FAIL("bandpass filter taps missing");
     dim3 grid(T * C, (S + BP_SEG - 1) / BP_SEG);
+    const size_t lds = (size_t)bp_entries(bp_window(L)) * sizeof(float2) + (L > 0 ? (size_t)(L + 46) * sizeof(float) : 0);
     if (type == TYPE_I16)
-        convert_demod_kernel<int16_t><<<grid, 256, 0, st>>>((const int16_t*)raw, d_map, R0, C, S, fd_fs, d_h, L, out);
+        convert_demod_kernel<int16_t><<<grid, 256, lds, st>>>((const int16_t*)raw, d_map, R0, C, S, fd_fs, d_h, L, out);
     else if (type == TYPE_F32)
-        convert_demod_kernel<float><<<grid, 256, 0, st>>>((const float*)raw, d_map, R0, C, S, fd_fs, d_h, L, out);
+        convert_demod_kernel<float><<<grid, 256, lds, st>>>((const float*)raw, d_map, R0, C, S, fd_fs, d_h, L, out);
     else
         CUT_FAIL("Data converter: Unsupported input data type.");
     CUT_HIP_CHECK(hipGetLastError());
