@@ -177,6 +177,7 @@ def sequences_leg(dev, stream, repeats=3):
     from cuda_ultrasound_toolkit_amd import _lib as L, api, configs
     lib = L.load()
     out = {}
+    assert api.set_das_options(cubic=True)  # the sequences' interpolation, as labelled below
     cases = {
         "forces_2d": (configs.linear_2d(seed=1, generate=False),
                       "FORCES: 32 tx x 32 ch x 1024 samples -> 128 x 256, cubic, no apodization"),

@@ -113,6 +113,20 @@ __device__ __forceinline__ float2 iq_at(const void* __restrict__ base, size_t i,
     }
 }
 
+// Cubic interpolation at i + t from an FP32 IQ buffer of n samples: p[i-1] ..
+// p[i+2], p[i+2] clamped to p[n-1] (1 <= i <= n-2), fetched as two 16-byte
+// loads of sample pairs (8-byte aligned; the hardware takes unaligned 16-byte
+// loads).  On the 128^3 RCA_TPW volume: 10.5 -> 10.1 ms against four 8-byte
+// loads.  (One 16-byte load for the linear pair measured no better.)
+__device__ __forceinline__ float2 cubic_pairs(const float2* __restrict__ iq, int i, int n, float t)
+{
+    float4 ab, cd;
+    __builtin_memcpy(&ab, iq + (i - 1), sizeof(float4));
+    __builtin_memcpy(&cd, iq + min(i + 1, n - 2), sizeof(float4));
+    if (i + 2 >= n) cd = make_float4(cd.z, cd.w, cd.z, cd.w);
+    return cubic2(make_float2(ab.x, ab.y), make_float2(ab.z, ab.w), make_float2(cd.x, cd.y), make_float2(cd.z, cd.w), t);
+}
+
 // Read p[i] of a row, index clamped into [0, S-1]: the reference's cubic form
 // touches p[S] at x = S-2 with weight exactly 0, any finite value works.
 template <int IQK>
@@ -1106,7 +1120,7 @@ __global__ __launch_bounds__(1024) void uforces_kernel(const float2* __restrict_
             const int b = row0 + e * p.S + clampi(i0 + (int)fl, 0, p.S - 2);
             float2 val;
             if constexpr (CUBIC)
-                val = cubic2(iq[b - 1], iq[b], iq[b + 1], iq[min(b + 2, n - 1)], tt);
+                val = cubic_pairs(iq, b, n, tt);
             else
                 val = lerp2(iq[b], iq[b + 1], tt);
             const float qx = val.x * w, qy = val.y * w;
@@ -1202,7 +1216,7 @@ __global__ __launch_bounds__(1024) void tpw_kernel(const float2* __restrict__ iq
                 // acts on non-finite input
                 const int i = row[j] + crow + clampi(i0[j] + (int)fl, 1, p.S - 2);
                 float2 val;
-                if constexpr (CUBIC) val = cubic2(iq[i - 1], iq[i], iq[i + 1], iq[min(i + 2, n - 1)], t);
+                if constexpr (CUBIC) val = cubic_pairs(iq, i, n, t);
                 else val = lerp2(iq[i], iq[i + 1], t);
                 const float ph = __builtin_amdgcn_fractf(fmaf(x, fcfs, ph0[j]));
                 const float cs = __builtin_amdgcn_cosf(ph), sn = __builtin_amdgcn_sinf(ph);
