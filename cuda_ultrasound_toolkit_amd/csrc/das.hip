@@ -966,6 +966,35 @@ __device__ __forceinline__ VoxelLane voxel_lane(long long nvox, int nw)
     return r;
 }
 
+// Voxel groups as 8 x 8 (x, y) patches of one depth plane (RCA_TPW): the 64
+// lanes of a group then read samples a few cache lines apart (the receive
+// delay of a column does not depend on y), where 64 voxels along x spread
+// their reads over ~8 lines.  Lanes past the grid's x or y edge compute a
+// clamped voxel and store nothing.
+inline long long patch_groups(int Nx, int Ny, int Nz) { return (long long)((Nx + 7) / 8) * ((Ny + 7) / 8) * Nz; }
+
+__device__ __forceinline__ VoxelLane voxel_lane_patch(int Nx, int Ny, int Nz, int nw, int& ix, int& iy, int& iz)
+{
+    VoxelLane r;
+    const int wb = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int per_block = (int)(blockDim.x >> 6) / nw;
+    const long long g = (long long)blockIdx.x * per_block + wb / nw;
+    const int npx = (Nx + 7) / 8, npy = (Ny + 7) / 8;
+    const long long pz = g / ((long long)npx * npy);
+    const int rem = (int)(g - pz * npx * npy);
+    const int py = rem / npx, px = rem - py * npx;
+    ix = px * 8 + (lane & 7);
+    iy = py * 8 + (lane >> 3);
+    r.valid = ix < Nx && iy < Ny && pz < Nz;
+    ix = min(ix, Nx - 1);
+    iy = min(iy, Ny - 1);
+    iz = (int)min(pz, (long long)Nz - 1);
+    r.v = ((long long)iz * Ny + iy) * Nx + ix;
+    r.w = wb % nw;
+    r.nw = nw;
+    return r;
+}
+
 __device__ __forceinline__ float2 apply_cf(float2 acc, float inc, float cw)
 {
     float cf = powf((acc.x * acc.x + acc.y * acc.y) / inc, cw);
@@ -1162,9 +1191,8 @@ template <bool CUBIC, int NW>
 __global__ __launch_bounds__(1024) void tpw_kernel(const float2* __restrict__ iq, const DasParams p,
                                                    const double2* __restrict__ sc, float2* __restrict__ vol)
 {
-    const VoxelLane vl = voxel_lane((long long)p.Nx * p.Ny * p.Nz, NW);
-    const long long v = vl.v;
-    const int ix = (int)(v % p.Nx), iy = (int)((v / p.Nx) % p.Ny), iz = (int)(v / ((long long)p.Nx * p.Ny));
+    int ix, iy, iz;
+    const VoxelLane vl = voxel_lane_patch(p.Nx, p.Ny, p.Nz, NW, ix, iy, iz);
     const double vxd = (double)p.min_x + ix * (double)p.res_x;
     const double vyd = (double)p.min_y + iy * (double)p.res_y;
     const double vzd = (double)p.min_z + iz * (double)p.res_z;
@@ -1486,7 +1514,9 @@ bool launch_das(const void* iq, int iq_fmt, float iq_scale, const DasParams& p, 
             CUT_HIP_CHECK(hipGetLastError());
         }
         auto run = [&](int nw) -> bool {
-            const unsigned grid = vox_grid(nvox, nw), block = vox_block(nw);
+            const unsigned grid = p.sequence == UFORCES ? vox_grid(nvox, nw)
+                                                        : vox_grid(64 * patch_groups(p.Nx, p.Ny, p.Nz), nw);
+            const unsigned block = vox_block(nw);
             with_waves(nw, [&](auto w) {
                 if (p.sequence == UFORCES) {
                     if (o.cubic) uforces_kernel<true, w()><<<grid, block, 0, st>>>(in, p, vol);
