@@ -966,12 +966,14 @@ __device__ __forceinline__ VoxelLane voxel_lane(long long nvox, int nw)
     return r;
 }
 
-// Voxel groups as 8 x 8 (x, y) patches of one depth plane (RCA_TPW): the 64
+// Voxel groups as 4 x 16 (x, y) patches of one depth plane (RCA_TPW): the 64
 // lanes of a group then read samples a few cache lines apart (the receive
-// delay of a column does not depend on y), where 64 voxels along x spread
-// their reads over ~8 lines.  Lanes past the grid's x or y edge compute a
-// clamped voxel and store nothing.
-inline long long patch_groups(int Nx, int Ny, int Nz) { return (long long)((Nx + 7) / 8) * ((Ny + 7) / 8) * Nz; }
+// delay of a column does not depend on y, the transmit delay moves slowly
+// with y), where 64 voxels along x spread their reads over ~8 lines.  On the
+// 128^3 volume 4 x 16 measured best (cubic / linear 8.3 / 5.5 ms; 8 x 8:
+// 8.6 / 5.9, 2 x 32: 8.15 / 5.7, 16 x 4: 8.8 / 6.4, 64 x 1: 10.1 / 6.4).
+// Lanes past the grid's x or y edge compute a clamped voxel and store nothing.
+inline long long patch_groups(int Nx, int Ny, int Nz) { return (long long)((Nx + 3) / 4) * ((Ny + 15) / 16) * Nz; }
 
 __device__ __forceinline__ VoxelLane voxel_lane_patch(int Nx, int Ny, int Nz, int nw, int& ix, int& iy, int& iz)
 {
@@ -979,12 +981,12 @@ __device__ __forceinline__ VoxelLane voxel_lane_patch(int Nx, int Ny, int Nz, in
     const int wb = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int per_block = (int)(blockDim.x >> 6) / nw;
     const long long g = (long long)blockIdx.x * per_block + wb / nw;
-    const int npx = (Nx + 7) / 8, npy = (Ny + 7) / 8;
+    const int npx = (Nx + 3) / 4, npy = (Ny + 15) / 16;
     const long long pz = g / ((long long)npx * npy);
     const int rem = (int)(g - pz * npx * npy);
     const int py = rem / npx, px = rem - py * npx;
-    ix = px * 8 + (lane & 7);
-    iy = py * 8 + (lane >> 3);
+    ix = px * 4 + (lane & 3);
+    iy = py * 16 + (lane >> 2);
     r.valid = ix < Nx && iy < Ny && pz < Nz;
     ix = min(ix, Nx - 1);
     iy = min(iy, Ny - 1);

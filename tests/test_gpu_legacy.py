@@ -50,7 +50,7 @@ CASES = {
     "uforces_2d_cw1_fn0": lambda: _with(configs.uforces_2d(seed=9, fn_rx=0.0), coherency_weighting=1.0),
     "tpw_3d_yz": lambda: configs.tpw_3d(seed=7),
     "tpw_3d_xz_cw05": lambda: _with(configs.tpw_3d(seed=8), rx_orientation=L.ORIENT_ROWS, coherency_weighting=0.5),
-    # x and y not multiples of the 8 x 8 voxel patches the kernel's waves take
+    # x and y not multiples of the 4 x 16 voxel patches the kernel's waves take
     "tpw_3d_ragged": lambda: _points(configs.tpw_3d(seed=10, n=21), (19, 13, 21)),
 }
 
